@@ -1,0 +1,272 @@
+/*
+ * hipns.h -- C ABI of the MI355X nested-sampling engine (libhipns.so).
+ *
+ * The library implements the sampling hot path of jaxns'
+ * ShardedStaticNestedSampler + UniDimSliceSampler as HIP kernels for gfx950.
+ * Every entry point below cites the reference function (jaxns 2.6.9,
+ * src/jaxns/<file>:<line>) whose work it replaces.
+ *
+ * Conventions
+ *  - Plain C: pointers, sizes and POD structs only.  Every pointer whose name
+ *    ends in _dev (or that sits in a *_dev struct field) is device memory owned
+ *    by the caller; the library never frees caller memory.
+ *  - Every call is stream-ordered on `stream` (a hipStream_t passed as void*,
+ *    NULL = default stream) and returns an int status: HIPNS_OK or a negative
+ *    HIPNS_E* code; hipns_last_error() gives the message for the calling thread.
+ *  - Scratch memory comes from the caller through (workspace, workspace_bytes);
+ *    the matching *_workspace_size() reports how much is needed.
+ *  - Calls are re-entrant for distinct buffers/workspaces.  No host threads.
+ */
+#ifndef HIPNS_H
+#define HIPNS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPNS_ABI_VERSION 1
+
+#define HIPNS_OK 0
+#define HIPNS_EINVAL (-1)  /* bad argument (null pointer, size, layout) */
+#define HIPNS_EHIP (-2)    /* HIP runtime error */
+#define HIPNS_ENOMEM (-3)  /* workspace too small */
+#define HIPNS_ERANGE (-4)  /* size beyond a documented kernel limit */
+#define HIPNS_ENOTSUP (-5) /* combination not supported */
+
+/* likelihood families fused into the sampling kernels */
+#define HIPNS_FAM_MVN 0    /* log N(x; mu, Sigma) on an affine prior: r = W q(u) + b, logL = c - |r|^2/2 */
+#define HIPNS_FAM_GMM 1    /* log sum_k w_k N(x; m_k, s_k^2 I), x = p + s (.) q(u)                          */
+#define HIPNS_FAM_LINREG 2 /* -N/2 log(2 pi s^2) - |y - X w|^2 / (2 s^2), w = p + s (.) q(u)                */
+
+/* maximum sizes of the fused kernels */
+#define HIPNS_MAX_DIM 128        /* padded U-space dimension */
+#define HIPNS_MAX_LIVE 8192      /* live points per run */
+#define HIPNS_MAX_GMM_K 8        /* mixture components */
+
+/* ---------------------------------------------------------------------
+ * Per-run description and mutable state (device arrays of these are
+ * indexed by run).  Offsets are element offsets into the packed buffers.
+ * ------------------------------------------------------------------- */
+typedef struct hipns_run_desc {
+  int32_t D;                /* U-space dimension */
+  int32_t N;                /* num_live_points */
+  int32_t shell;            /* shell size = int(N * shell_fraction) */
+  int32_t num_slices;       /* slices per chain (s * D) */
+  int32_t k_phantom;        /* phantom samples kept per chain (k) */
+  int32_t midpoint_shrink;  /* 1: midpoint (alpha) shrink, 0: shrink to rejected point */
+  int32_t sampler;          /* 0: UniDimSliceSampler, 1: UniformSampler (rejection) */
+  int32_t max_uniform_evals;/* UniformSampler cap (100 in the reference) */
+  int64_t max_samples;      /* sample-collection length */
+  int64_t live_off;         /* offset into the live-point arrays (points) */
+  int64_t samp_off;         /* offset into the sample-collection arrays (samples) */
+  int64_t chain_off;        /* offset into the new-sample arrays (chains) */
+  int64_t param_off;        /* offset into the model parameter buffer (floats) */
+  uint32_t seed_lo, seed_hi;/* Philox key of the run */
+  double log_const;         /* additive constant of the likelihood */
+} hipns_run_desc_t;
+
+typedef struct hipns_run_state {
+  int64_t next_idx;         /* NestedSamplerState.next_sample_idx */
+  int64_t num_samples;      /* NestedSamplerState.num_samples */
+  int64_t iteration;        /* shells collected */
+  int64_t num_likelihood_evaluations; /* TerminationRegister fields ... */
+  int64_t termination_reason;
+  int64_t num_samples_used;
+  double evidence_calc[8];  /* EvidenceCalculation: log_L, log_X, log_X2, log_Z, log_ZX, log_Z2, log_dZ, log_dZ2 */
+  double evidence_calc_with_remaining[8];
+  double log_L_contour;
+  double efficiency;
+  double relative_spread;
+  double absolute_spread;
+  double peak_log_XL;
+  double contour;           /* contour of the shell being collected */
+  int32_t plateau;
+  int32_t no_seed_points;
+  int32_t done;             /* 1 once the termination condition holds */
+  int32_t buf;              /* which live buffer (0/1) holds the current sorted live set */
+  int32_t first_seed;       /* index of the first live point with logL > contour */
+  int32_t sender_node;      /* sender node index of the shell's new samples */
+  int32_t flags;            /* diagnostic bits: 1 = a slice hit the step cap */
+  int32_t pad0;
+  int64_t cap_hits;         /* slices that hit the step cap (accepted the seed point) */
+} hipns_run_state_t;
+
+/* TerminationCondition (nested_samplers/common/types.py:26-54); a field is active when has_* != 0 */
+typedef struct hipns_term_cond {
+  int32_t has_ess, has_evidence_uncert, has_dlogZ, has_max_samples, has_max_num_likelihood_evaluations;
+  int32_t has_log_L_contour, has_efficiency_threshold, has_rtol, has_atol, has_peak_XL_frac;
+  double ess, evidence_uncert, dlogZ, max_samples, max_num_likelihood_evaluations;
+  double log_L_contour, efficiency_threshold, rtol, atol, peak_XL_frac;
+} hipns_term_cond_t;
+
+/* Device buffers of a batch of runs.  U rows have stride `dps` floats
+ * (dps = padded dimension, a multiple of 32, >= every run's D). */
+typedef struct hipns_buffers {
+  float* live_U[2];
+  double* live_logL[2];
+  double* live_cons[2];
+  int32_t* live_sender[2];
+  int32_t* live_evals[2];
+  float* samp_U;
+  double* samp_logL;
+  int32_t* samp_sender;
+  int32_t* samp_evals;
+  uint8_t* samp_phantom;
+  float* new_U;
+  double* new_logL;
+  int32_t* new_evals;
+  const float* params;      /* packed model parameters */
+  int32_t dps;
+  int32_t pad0;
+} hipns_buffers_t;
+
+/* Kernel shape picked by the autotuner: lanes per chain (2: 32 chains per
+ * wavefront with 32x32 MFMA tiles, 4: 16 chains per wavefront with 16x16 tiles),
+ * wavefronts per workgroup, and the likelihood family. */
+typedef struct hipns_launch {
+  int32_t family;
+  int32_t lanes_per_chain;
+  int32_t waves_per_block;
+  int32_t gmm_K;
+  int32_t lr_ndata;         /* LinReg: data rows */
+  int32_t max_steps;        /* step cap per slice (safety bound on the shrink loop) */
+} hipns_launch_t;
+
+/* ---------------------------------------------------------------------
+ * Library
+ * ------------------------------------------------------------------- */
+int hipns_abi_version(void);
+/* copies the calling thread's last error message (NUL-terminated) */
+int hipns_last_error(char* buf, size_t len);
+
+/* ---------------------------------------------------------------------
+ * (a)+(b)+(c)+(d): the shell loop.  R runs are advanced together; runs whose
+ * state says done are skipped.
+ * ------------------------------------------------------------------- */
+
+/* create_init_state + create_init_termination_register
+ * (nested_samplers/common/initialisation.py:20-108, uniform_sample.py:12-61):
+ * draws each run's N live points uniformly with logL > -inf, sorts them
+ * (stable argsort, initialisation.py:75-76) and clears the register. */
+int hipns_create_init_state(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                            const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs,
+                            const hipns_launch_t* launch, void* workspace, size_t workspace_bytes, void* stream);
+
+/* One _collect_shell per run (nested_samplers/sharded/sharded_static.py:210-324) followed
+ * by determine_termination (nested_samplers/common/termination.py:13-147):
+ * discard the lowest `shell` live points into the sample collection, run one
+ * sampler chain per discarded point from seeds above the contour
+ * (UniDimSliceSampler.get_sample, samplers/uni_slice_sampler.py:114-441, or
+ * UniformSampler, samplers/uniform_samplers.py:42-85), write the chains back,
+ * stable-sort the live set, append phantoms, update the evidence register and
+ * the termination flag. */
+int hipns_collect_shell(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                        const hipns_term_cond_t* term, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The four stages of hipns_collect_shell, exposed so that each can be timed:
+ *  discard  : _add_samples_to_state(live[:shell]) + contour/seed range/sender (sharded_static.py:237-261)
+ *  sample   : get_samples -> one UniDimSliceSampler (or UniformSampler) chain per discarded point
+ *             (sharded_static.py:88-129, samplers/uni_slice_sampler.py:114-441, bases.py:63-74)
+ *  replace  : live[:shell] = new; stable argsort; permute (sharded_static.py:262-279)
+ *  register : evidence register + termination flag (sharded_static.py:281-323, termination.py:13-147) */
+int hipns_shell_discard(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream);
+int hipns_shell_sample(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                       const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                       void* stream);
+int hipns_shell_replace(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream);
+int hipns_shell_register(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                         const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs,
+                         const hipns_term_cond_t* term, void* stream);
+
+/* Final `_add_samples_to_state(live_point_collection)` of _run
+ * (sharded_static.py:834-838): appends the live set to the sample collection. */
+int hipns_consume_live(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                       const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream);
+
+/* determine_termination on each run's current register without collecting a
+ * shell: the while_loop condition of _main_ns_thread (sharded_static.py:480-485)
+ * evaluated before the first shell of a phase.  set_no_seed != 0 first sets
+ * no_seed_points from the incoming live set (sharded_static.py:471-473). */
+int hipns_check_termination(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                            const hipns_buffers_t* bufs, const hipns_term_cond_t* term, int32_t set_no_seed,
+                            void* stream);
+
+size_t hipns_collect_shell_workspace_size(int32_t R, const hipns_run_desc_t* runs_host);
+
+/* ---------------------------------------------------------------------
+ * (d) evidence statistics
+ * ------------------------------------------------------------------- */
+
+/* compute_evidence_stats (internals/shrinkage_statistics.py:131-157): the
+ * _update_evidence_calc_op recurrence (:43-94) over M samples as a single-pass
+ * log-space scan.  per_sample_dev: [8][M] SoA (field-major, may be NULL);
+ * final_dev: [8].  stop_idx < 0 => static scan, else outputs past stop_idx hold
+ * the initial state (cumulative_op_dynamic, internals/cumulative_ops.py:82-130). */
+int hipns_evidence_scan(const double* log_L_dev, const double* num_live_points_dev, int64_t M, int64_t stop_idx,
+                        double* per_sample_dev, double* final_dev, void* workspace, size_t workspace_bytes,
+                        void* stream);
+size_t hipns_evidence_scan_workspace_size(int64_t M);
+
+/* count_crossed_edges (internals/tree_structure.py:33-108): stable sort of the
+ * N+1 node log-likelihoods (root -inf), out-degree by sender, prefix count.
+ * num_samples < 0 => static form. */
+int hipns_count_crossed_edges(const int32_t* sender_node_idx_dev, const double* log_L_dev, int64_t N,
+                              int64_t num_samples, int64_t* samples_indices_dev, int64_t* num_live_points_dev,
+                              void* workspace, size_t workspace_bytes, void* stream);
+size_t hipns_count_crossed_edges_workspace_size(int64_t N);
+
+/* Stable argsort of FP64 keys (jnp.argsort, stable=True; -inf < ... < +inf < NaN). */
+int hipns_argsort_stable(const double* keys_dev, int64_t n, int64_t* perm_dev, void* workspace,
+                         size_t workspace_bytes, void* stream);
+size_t hipns_argsort_workspace_size(int64_t n);
+
+/* ---------------------------------------------------------------------
+ * (b) batched prior transform and likelihood
+ * ------------------------------------------------------------------- */
+
+/* Model.forward over a batch (framework/model.py:167-176 -> framework/ops.py:302-326):
+ * logL_dev[i] = likelihood(prior_transform(U_dev[i, :D])) of run 0 of `runs_dev`,
+ * NaN -> -inf.  U rows have stride bufs->dps. */
+int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const hipns_run_desc_t* runs_dev,
+                         const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                         void* stream);
+
+/* Prior.forward of every dimension (framework/bases.py:58-68 ->
+ * framework/wrapped_tfp_distribution.py:77-84): X = loc + scale (.) q(U) for
+ * per-dimension transform q in {identity, ndtri}, optional dense lower factor. */
+int hipns_prior_transform(const float* U_dev, int64_t B, int32_t D, int32_t ldu, const float* loc_dev,
+                          const float* scale_dev, const float* tril_dev, const int32_t* qkind_dev, float* X_dev,
+                          int32_t ldx, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Parity hooks
+ * ------------------------------------------------------------------- */
+
+/* UniDimSliceSampler chains with injected draws (no PRNG): chain c starts at
+ * live point seed_idx_dev[c] of run 0 (live buffer `buf`), uses unit directions
+ * dirs_dev [C][num_slices][dps] and pick uniforms tu_dev [C][num_slices][max_steps],
+ * and records trace_t/trace_logL [C][num_slices][max_steps] and steps_dev
+ * [C][num_slices].  Outputs go to bufs->new_* (chains) and, for k > 0, the
+ * phantom arrays ph_U_dev [C][k][dps], ph_logL_dev [C][k]. */
+int hipns_slice_chains_injected(const hipns_run_desc_t* runs_dev, const hipns_run_desc_t* runs_host,
+                                const hipns_buffers_t* bufs, const hipns_launch_t* launch, int32_t buf,
+                                int32_t C, double contour, const int32_t* seed_idx_dev, const float* dirs_dev,
+                                const float* tu_dev, float* trace_t_dev, double* trace_logL_dev, int32_t* steps_dev,
+                                float* ph_U_dev, double* ph_logL_dev, void* stream);
+
+/* Philox4x32-10 blocks: out_dev[i] = philox(ctr_dev[i], key) (4 words each). */
+int hipns_philox4x32(const uint32_t* ctr_dev, int64_t n, uint32_t key0, uint32_t key1, uint32_t* out_dev,
+                     void* stream);
+
+/* Device inverse normal CDF (the prior transform's q = ndtri), for accuracy tests. */
+int hipns_ndtri_f32(const float* u_dev, int64_t n, float* z_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

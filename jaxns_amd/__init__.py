@@ -1,0 +1,23 @@
+"""jaxns_amd -- the jaxns nested-sampling hot path on AMD Instinct MI355X.
+
+Drop-in surface for jaxns' sampling path:
+    from jaxns_amd import Model, Prior, NestedSampler, DefaultNestedSampler, TerminationCondition
+    from jaxns_amd import distributions as tfpd
+    from jaxns_amd.likelihoods import MultivariateNormalLikelihood
+
+The sampling loop (slice chains, batched likelihoods, live-point replacement,
+evidence register) and to_results' tree count and evidence scan run as HIP
+kernels from libhipns.so (jaxns_amd/csrc); Python drives them through ctypes.
+"""
+from . import distributions
+from .framework import InvalidPriorName, Model, Prior
+from .likelihoods import GaussianMixtureLikelihood, LinearRegressionLikelihood, MultivariateNormalLikelihood
+from .nested_sampler import DefaultNestedSampler, NestedSampler, NestedSamplerState
+from .types import (NestedSamplerResults, TerminationCondition, TerminationConditionConjunction,
+                    TerminationConditionDisjunction)
+from .utils import load_results, save_results, summary
+
+__all__ = ["Model", "Prior", "NestedSampler", "DefaultNestedSampler", "NestedSamplerState", "TerminationCondition",
+           "TerminationConditionConjunction", "TerminationConditionDisjunction", "NestedSamplerResults",
+           "MultivariateNormalLikelihood", "GaussianMixtureLikelihood", "LinearRegressionLikelihood", "summary",
+           "save_results", "load_results", "distributions", "InvalidPriorName"]

@@ -1,0 +1,458 @@
+// api.hip -- the extern "C" entry points of libhipns (declared in include/hipns.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "hipns_dev.h"
+#include "ns_sampler.h"
+#include "ns_shell.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  return fail(HIPNS_EHIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+#define HIPNS_CHECK_LAUNCH(where)                \
+  do {                                           \
+    hipError_t _e = hipGetLastError();           \
+    if (_e != hipSuccess) return hip_fail(_e, where); \
+  } while (0)
+
+hipStream_t as_stream(void* s) { return (hipStream_t)s; }
+
+int check_buffers(const hipns_buffers_t* b) {
+  if (!b) return fail(HIPNS_EINVAL, "null buffers");
+  if (b->dps <= 0 || b->dps % 32 != 0 || b->dps > HIPNS_MAX_DIM)
+    return fail(HIPNS_EINVAL, "dps=%d must be a positive multiple of 32 and <= %d", b->dps, HIPNS_MAX_DIM);
+  return HIPNS_OK;
+}
+
+int check_runs(const hipns_run_desc_t* runs_host, int32_t R, const hipns_buffers_t* b, int* max_D, int* max_N,
+               int* max_shell) {
+  if (!runs_host || R <= 0) return fail(HIPNS_EINVAL, "need R > 0 run descriptors (host copy)");
+  *max_D = 0;
+  *max_N = 0;
+  *max_shell = 0;
+  for (int r = 0; r < R; ++r) {
+    const hipns_run_desc_t& d = runs_host[r];
+    if (d.D < 1 || d.D > b->dps) return fail(HIPNS_EINVAL, "run %d: D=%d outside [1, dps=%d]", r, d.D, b->dps);
+    if (d.N < 2 || d.N > HIPNS_MAX_LIVE) return fail(HIPNS_ERANGE, "run %d: N=%d outside [2, %d]", r, d.N, HIPNS_MAX_LIVE);
+    if (d.shell < 1 || d.shell > d.N) return fail(HIPNS_EINVAL, "run %d: shell=%d outside [1, N]", r, d.shell);
+    if (d.num_slices < 1) return fail(HIPNS_EINVAL, "run %d: num_slices must be >= 1", r);
+    if (d.k_phantom < 0 || d.k_phantom >= d.num_slices)
+      return fail(HIPNS_EINVAL, "run %d: need 0 <= k < num_slices", r);
+    if (d.max_samples < (int64_t)d.shell * (1 + d.k_phantom) + d.N)
+      return fail(HIPNS_EINVAL, "run %d: max_samples=%lld too small", r, (long long)d.max_samples);
+    if (d.max_samples > (int64_t)0x7fffffff) return fail(HIPNS_ERANGE, "run %d: max_samples beyond int32", r);
+    if (d.D > *max_D) *max_D = d.D;
+    if (d.N > *max_N) *max_N = d.N;
+    if (d.shell > *max_shell) *max_shell = d.shell;
+  }
+  return HIPNS_OK;
+}
+
+int check_launch(const hipns_launch_t* L) {
+  if (!L) return fail(HIPNS_EINVAL, "null launch config");
+  if (L->family != HIPNS_FAM_MVN && L->family != HIPNS_FAM_GMM)
+    return fail(HIPNS_ENOTSUP, "family %d not supported by the fused sampler", L->family);
+  if (L->lanes_per_chain != 2 && L->lanes_per_chain != 4)
+    return fail(HIPNS_EINVAL, "lanes_per_chain must be 2 or 4");
+  if (L->waves_per_block < 1 || L->waves_per_block > 4) return fail(HIPNS_EINVAL, "waves_per_block must be 1..4");
+  if (L->family == HIPNS_FAM_GMM && (L->gmm_K < 1 || L->gmm_K > HIPNS_MAX_GMM_K))
+    return fail(HIPNS_EINVAL, "gmm_K must be 1..%d", HIPNS_MAX_GMM_K);
+  if (L->max_steps < 2) return fail(HIPNS_EINVAL, "max_steps must be >= 2");
+  return HIPNS_OK;
+}
+
+int max_lds_bytes() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
+  return v;
+}
+
+int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A, int R, int max_chains, int maxD,
+                   int dps, hipStream_t s) {
+  int lpc = (dps > 64) ? 4 : L->lanes_per_chain;
+  size_t lds = hipns::sampler_lds_floats(L->family, dps, maxD, L->gmm_K, lpc) * sizeof(float);
+  if ((int)lds > max_lds_bytes())
+    return fail(HIPNS_ERANGE, "model parameters need %zu B of LDS, device allows %d", lds, max_lds_bytes());
+  hipns_launch_t L2 = *L;
+  L2.lanes_per_chain = lpc;
+  hipError_t e;
+  switch (mode) {
+    case hipns::MODE_SLICE: e = hipns::launch_sampler_slice(A, R, max_chains, L2, dps, lds, s); break;
+    case hipns::MODE_UNIF: e = hipns::launch_sampler_unif(A, R, max_chains, L2, dps, lds, s); break;
+    case hipns::MODE_INIT: e = hipns::launch_sampler_init(A, R, max_chains, L2, dps, lds, s); break;
+    default: e = hipns::launch_sampler_inj(A, R, max_chains, L2, dps, lds, s); break;
+  }
+  if (e != hipSuccess) return hip_fail(e, "sampler launch");
+  return HIPNS_OK;
+}
+
+int rows_blocks(int64_t rows, int dps) {
+  int64_t b = (rows * dps + 255) / 256;
+  if (b > 1024) b = 1024;
+  return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// kernels local to the API (test hooks and the prior transform)
+// ---------------------------------------------------------------------------
+__global__ void k_philox_fill(const uint32_t* ctr, int64_t n, uint32_t k0, uint32_t k1, uint32_t* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  hipns::u32x4 w = hipns::philox4x32(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3], k0, k1);
+  out[4 * i] = w.x;
+  out[4 * i + 1] = w.y;
+  out[4 * i + 2] = w.z;
+  out[4 * i + 3] = w.w;
+}
+
+__global__ void k_ndtri(const float* u, int64_t n, float* z) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = hipns::ndtri_f32(u[i]);
+}
+
+// X[b, i] = loc[i] + sum_{j<=i} tril[i, j] q_j(U[b, j]) when tril != NULL, else loc[i] + scale[i] q_i(U[b, i])
+__global__ void k_prior_transform(const float* U, int64_t B, int D, int ldu, const float* loc, const float* scale,
+                                  const float* tril, const int32_t* qkind, float* X, int ldx) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
+  if (b >= B) return;
+  for (int i = threadIdx.x; i < D; i += blockDim.x) {
+    float acc = loc[i];
+    if (tril) {
+      for (int j = 0; j <= i; ++j) {
+        float u = U[b * ldu + j];
+        float q = qkind[j] ? hipns::ndtri_f32(u) : u;
+        acc = fmaf(tril[(int64_t)i * D + j], q, acc);
+      }
+    } else {
+      float u = U[b * ldu + i];
+      float q = qkind[i] ? hipns::ndtri_f32(u) : u;
+      acc = fmaf(scale[i], q, acc);
+    }
+    X[b * ldx + i] = acc;
+  }
+}
+
+__global__ void k_widen_i32(const int32_t* in, int64_t n, int64_t* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int hipns_abi_version(void) { return HIPNS_ABI_VERSION; }
+
+int hipns_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return HIPNS_EINVAL;
+  size_t n = g_err.size() < len - 1 ? g_err.size() : len - 1;
+  memcpy(buf, g_err.data(), n);
+  buf[n] = 0;
+  return HIPNS_OK;
+}
+
+int hipns_create_init_state(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                            const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs,
+                            const hipns_launch_t* launch, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  int maxD, maxN, maxS;
+  if ((rc = check_runs(runs_host, R, bufs, &maxD, &maxN, &maxS))) return rc;
+  if ((rc = check_launch(launch))) return rc;
+  if (!runs_dev || !states_dev) return fail(HIPNS_EINVAL, "null device run/state arrays");
+  hipStream_t s = as_stream(stream);
+  int64_t max_ms = 0;
+  for (int r = 0; r < R; ++r) max_ms = runs_host[r].max_samples > max_ms ? runs_host[r].max_samples : max_ms;
+  hipLaunchKernelGGL(hipns::k_init_state, dim3((R + 63) / 64), dim3(64), 0, s, states_dev, R);
+  HIPNS_CHECK_LAUNCH("k_init_state");
+  hipLaunchKernelGGL(hipns::k_init_samples, dim3(rows_blocks(max_ms, bufs->dps), R), dim3(256), 0, s, runs_dev, R,
+                     *bufs);
+  HIPNS_CHECK_LAUNCH("k_init_samples");
+  hipns::SampleArgs A = {};
+  A.runs = runs_dev;
+  A.states = states_dev;
+  A.B = *bufs;
+  A.max_steps = launch->max_steps;
+  A.gmm_K = launch->gmm_K;
+  if ((rc = sampler_launch(hipns::MODE_INIT, launch, A, R, maxN, maxD, bufs->dps, s))) return rc;
+  size_t lds = hipns::replace_lds_bytes(maxN);
+  if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "N=%d needs %zu B LDS for the sort", maxN, lds);
+  hipLaunchKernelGGL(hipns::k_replace, dim3(R), dim3(1024), lds, s, runs_dev, states_dev, *bufs, 1);
+  HIPNS_CHECK_LAUNCH("k_replace(init)");
+  (void)workspace;
+  (void)workspace_bytes;
+  return HIPNS_OK;
+}
+
+size_t hipns_collect_shell_workspace_size(int32_t R, const hipns_run_desc_t* runs_host) {
+  (void)R;
+  (void)runs_host;
+  return 0;
+}
+
+static int shell_common(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                        int* maxD, int* maxN, int* maxS) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  if ((rc = check_runs(runs_host, R, bufs, maxD, maxN, maxS))) return rc;
+  if (launch && (rc = check_launch(launch))) return rc;
+  if (!runs_dev || !states_dev) return fail(HIPNS_EINVAL, "null device run/state arrays");
+  return HIPNS_OK;
+}
+
+int hipns_shell_discard(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream) {
+  int rc, maxD, maxN, maxS;
+  if ((rc = shell_common(runs_dev, states_dev, R, runs_host, bufs, nullptr, &maxD, &maxN, &maxS))) return rc;
+  hipLaunchKernelGGL(hipns::k_discard, dim3(rows_blocks(maxS, bufs->dps), R), dim3(256), 0, as_stream(stream),
+                     runs_dev, states_dev, *bufs);
+  HIPNS_CHECK_LAUNCH("k_discard");
+  return HIPNS_OK;
+}
+
+int hipns_shell_sample(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                       const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                       void* stream) {
+  int rc, maxD, maxN, maxS;
+  if ((rc = shell_common(runs_dev, states_dev, R, runs_host, bufs, launch, &maxD, &maxN, &maxS))) return rc;
+  const int sampler = runs_host[0].sampler;
+  for (int r = 1; r < R; ++r)
+    if (runs_host[r].sampler != sampler) return fail(HIPNS_EINVAL, "all runs of a batch must use the same sampler");
+  hipns::SampleArgs A = {};
+  A.runs = runs_dev;
+  A.states = states_dev;
+  A.B = *bufs;
+  A.max_steps = launch->max_steps;
+  A.gmm_K = launch->gmm_K;
+  return sampler_launch(sampler == 1 ? hipns::MODE_UNIF : hipns::MODE_SLICE, launch, A, R, maxS, maxD, bufs->dps,
+                        as_stream(stream));
+}
+
+int hipns_shell_replace(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream) {
+  int rc, maxD, maxN, maxS;
+  if ((rc = shell_common(runs_dev, states_dev, R, runs_host, bufs, nullptr, &maxD, &maxN, &maxS))) return rc;
+  size_t lds = hipns::replace_lds_bytes(maxS) + (size_t)maxN * 4;
+  if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "N=%d needs %zu B LDS for the sort", maxN, lds);
+  hipLaunchKernelGGL(hipns::k_replace, dim3(R), dim3(1024), lds, as_stream(stream), runs_dev, states_dev, *bufs, 0);
+  HIPNS_CHECK_LAUNCH("k_replace");
+  return HIPNS_OK;
+}
+
+int hipns_shell_register(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                         const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs,
+                         const hipns_term_cond_t* term, void* stream) {
+  int rc, maxD, maxN, maxS;
+  if ((rc = shell_common(runs_dev, states_dev, R, runs_host, bufs, nullptr, &maxD, &maxN, &maxS))) return rc;
+  if (!term) return fail(HIPNS_EINVAL, "null termination condition");
+  hipLaunchKernelGGL(hipns::k_register, dim3(R), dim3(256), 0, as_stream(stream), runs_dev, states_dev, *bufs, *term);
+  HIPNS_CHECK_LAUNCH("k_register");
+  return HIPNS_OK;
+}
+
+int hipns_collect_shell(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                        const hipns_term_cond_t* term, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc;
+  if (!term) return fail(HIPNS_EINVAL, "null termination condition");
+  if ((rc = hipns_shell_discard(runs_dev, states_dev, R, runs_host, bufs, stream))) return rc;
+  if ((rc = hipns_shell_sample(runs_dev, states_dev, R, runs_host, bufs, launch, stream))) return rc;
+  if ((rc = hipns_shell_replace(runs_dev, states_dev, R, runs_host, bufs, stream))) return rc;
+  if ((rc = hipns_shell_register(runs_dev, states_dev, R, runs_host, bufs, term, stream))) return rc;
+  (void)workspace;
+  (void)workspace_bytes;
+  return HIPNS_OK;
+}
+
+int hipns_consume_live(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                       const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  int maxD, maxN, maxS;
+  if ((rc = check_runs(runs_host, R, bufs, &maxD, &maxN, &maxS))) return rc;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(hipns::k_consume, dim3(rows_blocks(maxN, bufs->dps), R), dim3(256), 0, s, runs_dev, states_dev,
+                     *bufs);
+  HIPNS_CHECK_LAUNCH("k_consume");
+  hipLaunchKernelGGL(hipns::k_consume_advance, dim3((R + 63) / 64), dim3(64), 0, s, runs_dev, states_dev, R);
+  HIPNS_CHECK_LAUNCH("k_consume_advance");
+  return HIPNS_OK;
+}
+
+int hipns_check_termination(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                                 const hipns_buffers_t* bufs, const hipns_term_cond_t* term, int32_t set_no_seed,
+                                 void* stream) {
+  if (!runs_dev || !states_dev || !term || !bufs || R <= 0) return fail(HIPNS_EINVAL, "bad arguments");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(hipns::k_check_term, dim3((R + 63) / 64), dim3(64), 0, s, runs_dev, states_dev, R, *term,
+                     set_no_seed, *bufs);
+  HIPNS_CHECK_LAUNCH("k_check_term");
+  return HIPNS_OK;
+}
+
+size_t hipns_evidence_scan_workspace_size(int64_t M) { return hipns::evidence_scan_ws(M); }
+
+int hipns_evidence_scan(const double* log_L_dev, const double* num_live_points_dev, int64_t M, int64_t stop_idx,
+                        double* per_sample_dev, double* final_dev, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+  if (M < 0) return fail(HIPNS_EINVAL, "M < 0");
+  if (M > 0 && (!log_L_dev || !num_live_points_dev)) return fail(HIPNS_EINVAL, "null inputs");
+  if (!per_sample_dev && !final_dev) return fail(HIPNS_EINVAL, "no outputs");
+  if (M >= ((int64_t)1 << 32) * 2048) return fail(HIPNS_ERANGE, "M too large");
+  if (workspace_bytes < hipns::evidence_scan_ws(M) || !workspace)
+    return fail(HIPNS_ENOMEM, "workspace %zu < %zu", workspace_bytes, hipns::evidence_scan_ws(M));
+  hipError_t e = hipns::evidence_scan(log_L_dev, num_live_points_dev, M, stop_idx, per_sample_dev, final_dev,
+                                      workspace, workspace_bytes, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "evidence_scan");
+  return HIPNS_OK;
+}
+
+size_t hipns_count_crossed_edges_workspace_size(int64_t N) { return hipns::count_crossed_edges_ws(N); }
+
+int hipns_count_crossed_edges(const int32_t* sender_node_idx_dev, const double* log_L_dev, int64_t N,
+                              int64_t num_samples, int64_t* samples_indices_dev, int64_t* num_live_points_dev,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+  if (N < 0 || N >= 0x7ffffffe) return fail(HIPNS_ERANGE, "N out of range");
+  if (num_samples > N) return fail(HIPNS_EINVAL, "num_samples > N");
+  if (N > 0 && (!sender_node_idx_dev || !log_L_dev || !samples_indices_dev || !num_live_points_dev))
+    return fail(HIPNS_EINVAL, "null pointer");
+  if (!workspace || workspace_bytes < hipns::count_crossed_edges_ws(N))
+    return fail(HIPNS_ENOMEM, "workspace %zu < %zu", workspace_bytes, hipns::count_crossed_edges_ws(N));
+  hipError_t e = hipns::count_crossed_edges(sender_node_idx_dev, log_L_dev, N, num_samples, samples_indices_dev,
+                                            num_live_points_dev, workspace, workspace_bytes, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "count_crossed_edges");
+  return HIPNS_OK;
+}
+
+size_t hipns_argsort_workspace_size(int64_t n) { return hipns::argsort_ws(n) + (size_t)n * 4 + 256; }
+
+int hipns_argsort_stable(const double* keys_dev, int64_t n, int64_t* perm_dev, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+  if (n < 0 || n >= 0x7fffffff) return fail(HIPNS_ERANGE, "n out of range");
+  if (n == 0) return HIPNS_OK;
+  if (!keys_dev || !perm_dev) return fail(HIPNS_EINVAL, "null pointer");
+  if (!workspace || workspace_bytes < hipns_argsort_workspace_size(n))
+    return fail(HIPNS_ENOMEM, "workspace too small");
+  int32_t* p32 = (int32_t*)workspace;
+  void* ws = (void*)(((uintptr_t)(p32 + n) + 255) & ~(uintptr_t)255);
+  hipStream_t s = as_stream(stream);
+  hipError_t e = hipns::argsort_stable(keys_dev, n, p32, ws, hipns::argsort_ws(n), s);
+  if (e != hipSuccess) return hip_fail(e, "argsort");
+  hipLaunchKernelGGL(k_widen_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p32, n, perm_dev);
+  HIPNS_CHECK_LAUNCH("k_widen_i32");
+  return HIPNS_OK;
+}
+
+int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const hipns_run_desc_t* runs_dev,
+                         const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                         void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  if ((rc = check_launch(launch))) return rc;
+  if (!runs_host || !runs_dev) return fail(HIPNS_EINVAL, "null runs");
+  if (B < 0) return fail(HIPNS_EINVAL, "B < 0");
+  if (B == 0) return HIPNS_OK;
+  if (!U_dev || !logL_dev) return fail(HIPNS_EINVAL, "null pointer");
+  const int D = runs_host[0].D;
+  if (D < 1 || D > bufs->dps) return fail(HIPNS_EINVAL, "D outside [1, dps]");
+  int lpc = bufs->dps > 64 ? 4 : launch->lanes_per_chain;
+  size_t lds = hipns::sampler_lds_floats(launch->family, bufs->dps, D, launch->gmm_K, lpc) * sizeof(float);
+  if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "parameters exceed LDS");
+  hipns_launch_t L2 = *launch;
+  L2.lanes_per_chain = lpc;
+  hipError_t e = hipns::launch_loglik(U_dev, B, logL_dev, runs_dev, *bufs, L2, bufs->dps, lds, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "loglik launch");
+  return HIPNS_OK;
+}
+
+int hipns_prior_transform(const float* U_dev, int64_t B, int32_t D, int32_t ldu, const float* loc_dev,
+                          const float* scale_dev, const float* tril_dev, const int32_t* qkind_dev, float* X_dev,
+                          int32_t ldx, void* stream) {
+  if (B < 0 || D < 1 || ldu < D || ldx < D) return fail(HIPNS_EINVAL, "bad sizes");
+  if (B == 0) return HIPNS_OK;
+  if (!U_dev || !loc_dev || !qkind_dev || !X_dev || (!tril_dev && !scale_dev)) return fail(HIPNS_EINVAL, "null");
+  dim3 block(64, 4);
+  hipLaunchKernelGGL(k_prior_transform, dim3((unsigned)((B + 3) / 4)), block, 0, as_stream(stream), U_dev, B, D, ldu,
+                     loc_dev, scale_dev, tril_dev, qkind_dev, X_dev, ldx);
+  HIPNS_CHECK_LAUNCH("k_prior_transform");
+  return HIPNS_OK;
+}
+
+int hipns_slice_chains_injected(const hipns_run_desc_t* runs_dev, const hipns_run_desc_t* runs_host,
+                                const hipns_buffers_t* bufs, const hipns_launch_t* launch, int32_t buf, int32_t C,
+                                double contour, const int32_t* seed_idx_dev, const float* dirs_dev,
+                                const float* tu_dev, float* trace_t_dev, double* trace_logL_dev, int32_t* steps_dev,
+                                float* ph_U_dev, double* ph_logL_dev, void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  int maxD, maxN, maxS;
+  if ((rc = check_runs(runs_host, 1, bufs, &maxD, &maxN, &maxS))) return rc;
+  if ((rc = check_launch(launch))) return rc;
+  if (C < 1 || C > runs_host[0].N) return fail(HIPNS_EINVAL, "C outside [1, N] (new-sample capacity)");
+  if (buf != 0 && buf != 1) return fail(HIPNS_EINVAL, "buf must be 0 or 1");
+  if (!seed_idx_dev || !dirs_dev || !tu_dev || !trace_t_dev || !trace_logL_dev || !steps_dev)
+    return fail(HIPNS_EINVAL, "null pointer");
+  if (runs_host[0].k_phantom > 0 && (!ph_U_dev || !ph_logL_dev)) return fail(HIPNS_EINVAL, "null phantom outputs");
+  hipns::SampleArgs A = {};
+  A.runs = runs_dev;
+  A.states = nullptr;
+  A.B = *bufs;
+  A.max_steps = launch->max_steps;
+  A.gmm_K = launch->gmm_K;
+  A.inj_buf = buf;
+  A.inj_C = C;
+  A.inj_contour = contour;
+  A.inj_seed = seed_idx_dev;
+  A.inj_dirs = dirs_dev;
+  A.inj_tu = tu_dev;
+  A.trace_t = trace_t_dev;
+  A.trace_logL = trace_logL_dev;
+  A.trace_steps = steps_dev;
+  A.ph_U = ph_U_dev;
+  A.ph_logL = ph_logL_dev;
+  return sampler_launch(hipns::MODE_INJ, launch, A, 1, C, maxD, bufs->dps, as_stream(stream));
+}
+
+int hipns_philox4x32(const uint32_t* ctr_dev, int64_t n, uint32_t key0, uint32_t key1, uint32_t* out_dev,
+                     void* stream) {
+  if (n < 0) return fail(HIPNS_EINVAL, "n < 0");
+  if (n == 0) return HIPNS_OK;
+  if (!ctr_dev || !out_dev) return fail(HIPNS_EINVAL, "null pointer");
+  hipLaunchKernelGGL(k_philox_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), ctr_dev, n,
+                     key0, key1, out_dev);
+  HIPNS_CHECK_LAUNCH("k_philox_fill");
+  return HIPNS_OK;
+}
+
+int hipns_ndtri_f32(const float* u_dev, int64_t n, float* z_dev, void* stream) {
+  if (n < 0) return fail(HIPNS_EINVAL, "n < 0");
+  if (n == 0) return HIPNS_OK;
+  if (!u_dev || !z_dev) return fail(HIPNS_EINVAL, "null pointer");
+  hipLaunchKernelGGL(k_ndtri, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), u_dev, n, z_dev);
+  HIPNS_CHECK_LAUNCH("k_ndtri");
+  return HIPNS_OK;
+}
+
+}  // extern "C"

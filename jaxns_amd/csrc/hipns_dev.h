@@ -1,0 +1,397 @@
+// hipns_dev.h -- device helpers shared by the gfx950 kernels of libhipns.
+//
+// Wavefront-64 idioms, the Philox4x32-10 counter PRNG, the inverse normal
+// CDF used by the prior transform, FP64 log-space arithmetic and the
+// composition algebra of the evidence recurrence.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hipns.h"
+
+#define HIPNS_WAVE 64
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace hipns {
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// ---------------------------------------------------------------------------
+// cross-lane reductions among the LPC lanes that share one chain.  Lane l
+// serves chain (l % CW) with part h = l / CW, CW = 64 / LPC, so partners are
+// at xor distances CW, 2 CW, ...
+// ---------------------------------------------------------------------------
+template <int LPC>
+__device__ __forceinline__ float chain_sum(float v) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+#pragma unroll
+  for (int o = CW; o < HIPNS_WAVE; o <<= 1) v += __shfl_xor(v, o, HIPNS_WAVE);
+  return v;
+}
+template <int LPC>
+__device__ __forceinline__ float chain_min(float v) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+#pragma unroll
+  for (int o = CW; o < HIPNS_WAVE; o <<= 1) v = fminf(v, __shfl_xor(v, o, HIPNS_WAVE));
+  return v;
+}
+template <int LPC>
+__device__ __forceinline__ float chain_max(float v) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+#pragma unroll
+  for (int o = CW; o < HIPNS_WAVE; o <<= 1) v = fmaxf(v, __shfl_xor(v, o, HIPNS_WAVE));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of the
+// multiply-xor bijection with Weyl key schedule.
+// ---------------------------------------------------------------------------
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                            uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return {c0, c1, c2, c3};
+}
+
+// uniform in [0,1) with 24 random bits
+__device__ __forceinline__ float u01f(uint32_t w) { return (float)(w >> 8) * 5.9604644775390625e-8f; }
+// uniform in (0,1]
+__device__ __forceinline__ float u01f_open0(uint32_t w) { return (float)((w >> 8) + 1u) * 5.9604644775390625e-8f; }
+// uniform in [0,1) with 53 random bits
+__device__ __forceinline__ double u01d(uint32_t a, uint32_t b) {
+  uint64_t v = (((uint64_t)a << 21) ^ (uint64_t)(b >> 11)) & ((1ull << 53) - 1);
+  return (double)v * (1.0 / 9007199254740992.0);
+}
+
+// counter-domain tags (word 3 of the Philox counter)
+enum : uint32_t {
+  RNG_SEED = 0u << 24,
+  RNG_DIR = 1u << 24,
+  RNG_PICK = 2u << 24,
+  RNG_UNIF = 3u << 24,
+};
+
+// ---------------------------------------------------------------------------
+// Inverse standard-normal CDF.
+//  - binary64: Wichura, Algorithm AS 241 (Applied Statistics 37, 1988), PPND16.
+//  - binary32: M. Giles, "Approximating the erfinv function" (GPU Computing
+//    Gems Jade, 2011), single-precision polynomials in w = -log(4u(1-u)),
+//    evaluated directly in u so that both tails keep relative accuracy;
+//    |u - 1/2| > 1/2 - 1e-7 falls back to PPND16.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ndtri_f64(double p) {
+  if (!(p > 0.0 && p < 1.0)) {
+    if (p == 0.0) return -__builtin_inf();
+    if (p == 1.0) return __builtin_inf();
+    return __builtin_nan("");
+  }
+  double q = p - 0.5;
+  if (fabs(q) <= 0.425) {
+    double r = 0.180625 - q * q;
+    double num = 2.5090809287301226727e+3;
+    num = num * r + 3.3430575583588128105e+4;
+    num = num * r + 6.7265770927008700853e+4;
+    num = num * r + 4.5921953931549871457e+4;
+    num = num * r + 1.3731693765509461125e+4;
+    num = num * r + 1.9715909503065514427e+3;
+    num = num * r + 1.3314166789178437745e+2;
+    num = num * r + 3.3871328727963666080e0;
+    double den = 5.2264952788528545610e+3;
+    den = den * r + 2.8729085735721942674e+4;
+    den = den * r + 3.9307895800092710610e+4;
+    den = den * r + 2.1213794301586595867e+4;
+    den = den * r + 5.3941960214247511077e+3;
+    den = den * r + 6.8718700749205790830e+2;
+    den = den * r + 4.2313330701600911252e+1;
+    den = den * r + 1.0;
+    return q * num / den;
+  }
+  double r = q < 0.0 ? p : 1.0 - p;
+  r = sqrt(-log(r));
+  double num, den;
+  if (r <= 5.0) {
+    r -= 1.6;
+    num = 7.74545014278341407640e-4;
+    num = num * r + 2.27238449892691845833e-2;
+    num = num * r + 2.41780725177450611770e-1;
+    num = num * r + 1.27045825245236838258e0;
+    num = num * r + 3.64784832476320460504e0;
+    num = num * r + 5.76949722146069140550e0;
+    num = num * r + 4.63033784615654529590e0;
+    num = num * r + 1.42343711074968357734e0;
+    den = 1.05075007164441684324e-9;
+    den = den * r + 5.47593808499534494600e-4;
+    den = den * r + 1.51986665636164571966e-2;
+    den = den * r + 1.48103976427480074590e-1;
+    den = den * r + 6.89767334985100004550e-1;
+    den = den * r + 1.67638483018380384940e0;
+    den = den * r + 2.05319162663775882187e0;
+    den = den * r + 1.0;
+  } else {
+    r -= 5.0;
+    num = 2.01033439929228813265e-7;
+    num = num * r + 2.71155556874348757815e-5;
+    num = num * r + 1.24266094738807843860e-3;
+    num = num * r + 2.65321895265761230930e-2;
+    num = num * r + 2.96560571828504891230e-1;
+    num = num * r + 1.78482653991729133580e0;
+    num = num * r + 5.46378491116411436990e0;
+    num = num * r + 6.65790464350110377720e0;
+    den = 2.04426310338993978564e-15;
+    den = den * r + 1.42151175831644588870e-7;
+    den = den * r + 1.84631831751005468180e-5;
+    den = den * r + 7.86869131145613259100e-4;
+    den = den * r + 1.48753612908506148525e-2;
+    den = den * r + 1.36929880922735805310e-1;
+    den = den * r + 5.99832206555887937690e-1;
+    den = den * r + 1.0;
+  }
+  double v = num / den;
+  return q < 0.0 ? -v : v;
+}
+
+__device__ __noinline__ float ndtri_tail_f32(float u) { return (float)ndtri_f64((double)u); }
+
+__device__ __forceinline__ float ndtri_f32(float u) {
+  // far tails (and u outside (0,1)): binary64 AS 241
+  if (!(u >= 1e-7f && u <= 0.9999999f)) return ndtri_tail_f32(u);
+  float w = -__logf(4.0f * u * (1.0f - u));
+  float x = 2.0f * u - 1.0f;
+  float p;
+  if (w < 5.0f) {
+    w = w - 2.5f;
+    p = 2.81022636e-08f;
+    p = fmaf(p, w, 3.43273939e-07f);
+    p = fmaf(p, w, -3.5233877e-06f);
+    p = fmaf(p, w, -4.39150654e-06f);
+    p = fmaf(p, w, 0.00021858087f);
+    p = fmaf(p, w, -0.00125372503f);
+    p = fmaf(p, w, -0.00417768164f);
+    p = fmaf(p, w, 0.246640727f);
+    p = fmaf(p, w, 1.50140941f);
+  } else {
+    w = sqrtf(w) - 3.0f;
+    p = -0.000200214257f;
+    p = fmaf(p, w, 0.000100950558f);
+    p = fmaf(p, w, 0.00134934322f);
+    p = fmaf(p, w, -0.00367342844f);
+    p = fmaf(p, w, 0.00573950773f);
+    p = fmaf(p, w, -0.0076224613f);
+    p = fmaf(p, w, 0.00943887047f);
+    p = fmaf(p, w, 1.00167406f);
+    p = fmaf(p, w, 2.83297682f);
+  }
+  return 1.41421356237309505f * p * x;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 log-space arithmetic.  lae(a, b) = log(e^a + e^b) with the jnp.logaddexp
+// convention for equal infinities (returns a + b).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lae(double a, double b) {
+  double m = fmax(a, b);
+  double d = a - b;
+  if (__builtin_isnan(d)) return a + b;
+  return m + log1p(exp(-fabs(d)));
+}
+
+// ---------------------------------------------------------------------------
+// Evidence recurrence as composable maps.
+//
+// One _update_evidence_calc_op step (internals/shrinkage_statistics.py:43-94)
+// acts on S = (X, X2, Z, ZX, Z2, dZ2) (all >= 0, kept as logs) linearly:
+//   X'  = T X                 X2' = T2 X2
+//   Z'  = Z  + (t m) X        ZX' = T ZX + (tT m) X2
+//   Z2' = Z2 + (2 t m) ZX + (t2 m^2) X2
+//   dZ2'= dZ2 + (t2 m^2) X2
+// with T=n/(n+1), t=1/(n+1), T2=n/(n+2), t2=2/((n+1)(n+2)), tT=n/((n+1)(n+2)),
+// m = (L_prev + L)/2.  Maps of the form
+//   X'=aX, X2'=bX2, Z'=Z+gX, ZX'=aZX+eX2, Z2'=Z2+zZX+hX2, dZ2'=dZ2+thX2
+// are closed under composition, so the scan over samples is a prefix scan
+// of 7-coefficient maps in log space.
+// ---------------------------------------------------------------------------
+struct EvMap {
+  double a, b, g, e, z, h, th;
+};
+
+struct EvState {
+  double L, X, X2, Z, ZX, Z2, dZ, dZ2;
+};
+
+__device__ __forceinline__ EvMap evmap_identity() {
+  const double ninf = -__builtin_inf();
+  return {0.0, 0.0, ninf, ninf, ninf, ninf, ninf};
+}
+
+// the single-step map of sample (n, logL) preceded by log-likelihood Lprev; the
+// per-factor expressions follow shrinkage_statistics.py:47-72.
+__device__ __forceinline__ EvMap evmap_step(double n, double Lprev, double L, double* log_tm) {
+  const double LOG2 = 0.69314718055994530942;
+  double midL = -LOG2 + lae(L, Lprev);
+  double log_n = log(n);
+  double log_n1 = log(n + 1.0);
+  double log_n2 = log(n + 2.0);
+  double T = -lae(0.0, -log_n);
+  double t = -log_n1;
+  double T2 = -lae(0.0, LOG2 - log_n);
+  double t2 = LOG2 - log_n1 - log_n2;
+  double tT = T - log_n2;
+  EvMap m;
+  m.a = T;
+  m.b = T2;
+  m.g = t + midL;
+  m.e = tT + midL;
+  m.z = LOG2 + t + midL;
+  m.h = t2 + 2.0 * midL;
+  m.th = m.h;
+  *log_tm = m.g;
+  return m;
+}
+
+// f then g
+__device__ __forceinline__ EvMap evmap_then(const EvMap& f, const EvMap& g) {
+  EvMap r;
+  r.a = g.a + f.a;
+  r.b = g.b + f.b;
+  r.g = lae(f.g, g.g + f.a);
+  r.e = lae(g.a + f.e, g.e + f.b);
+  r.z = lae(f.z, g.z + f.a);
+  r.h = lae(lae(f.h, g.z + f.e), g.h + f.b);
+  r.th = lae(f.th, g.th + f.b);
+  return r;
+}
+
+__device__ __forceinline__ EvState evmap_apply(const EvMap& f, const EvState& s) {
+  EvState o;
+  o.L = s.L;
+  o.X = f.a + s.X;
+  o.X2 = f.b + s.X2;
+  o.Z = lae(s.Z, f.g + s.X);
+  o.ZX = lae(f.a + s.ZX, f.e + s.X2);
+  o.Z2 = lae(lae(s.Z2, f.z + s.ZX), f.h + s.X2);
+  o.dZ = s.dZ;
+  o.dZ2 = lae(s.dZ2, f.th + s.X2);
+  return o;
+}
+
+// one sequential recurrence step (the reference op itself); returns the new state
+__device__ __forceinline__ EvState ev_step(const EvState& s, double n, double L) {
+  double log_tm;
+  EvMap f = evmap_step(n, s.L, L, &log_tm);
+  EvState o = evmap_apply(f, s);
+  o.L = L;
+  o.dZ = log_tm + s.X;
+  return o;
+}
+
+__device__ __forceinline__ EvState evstate_from(const double* s8) {
+  return {s8[0], s8[1], s8[2], s8[3], s8[4], s8[5], s8[6], s8[7]};
+}
+__device__ __forceinline__ void evstate_to(const EvState& s, double* s8) {
+  s8[0] = s.L;
+  s8[1] = s.X;
+  s8[2] = s.X2;
+  s8[3] = s.Z;
+  s8[4] = s.ZX;
+  s8[5] = s.Z2;
+  s8[6] = s.dZ;
+  s8[7] = s.dZ2;
+}
+
+__device__ __forceinline__ EvMap shfl_xor_map(const EvMap& m, int o) {
+  return {__shfl_xor(m.a, o, 64), __shfl_xor(m.b, o, 64), __shfl_xor(m.g, o, 64), __shfl_xor(m.e, o, 64),
+          __shfl_xor(m.z, o, 64), __shfl_xor(m.h, o, 64), __shfl_xor(m.th, o, 64)};
+}
+__device__ __forceinline__ EvMap shfl_up_map(const EvMap& m, int o) {
+  return {__shfl_up(m.a, o, 64), __shfl_up(m.b, o, 64), __shfl_up(m.g, o, 64), __shfl_up(m.e, o, 64),
+          __shfl_up(m.z, o, 64), __shfl_up(m.h, o, 64), __shfl_up(m.th, o, 64)};
+}
+
+// linear_to_log_stats (internals/stats.py:55-74)
+__device__ __forceinline__ void linear_to_log_stats(double log_f, double log_f2, double* mu, double* var) {
+  double fsq = log_f + log_f;
+  *mu = fsq - 0.5 * log_f2;
+  double s2 = log_f2 - fsq;
+  const double eps = 2.220446049250313e-16;
+  *var = __builtin_isnan(s2) ? s2 : fmax(s2, eps);
+}
+
+// determine_termination for one TerminationCondition (termination.py:67-147)
+__device__ __forceinline__ int determine_termination(const hipns_term_cond_t& c, const hipns_run_state_t& r,
+                                                     int64_t* reason) {
+  int done = 0;
+  int64_t why = 0;
+  auto set = [&](bool cond, int bit) {
+    if (cond) {
+      done = 1;
+      why += (int64_t)1 << bit;
+    }
+  };
+  if (c.has_max_samples) set((double)r.num_samples_used >= c.max_samples, 0);
+  if (c.has_evidence_uncert) {
+    double mu, var;
+    linear_to_log_stats(r.evidence_calc_with_remaining[3], r.evidence_calc_with_remaining[5], &mu, &var);
+    set(var <= c.evidence_uncert * c.evidence_uncert, 1);
+  }
+  if (c.has_dlogZ) {
+    double mu1, v1, mu0, v0;
+    linear_to_log_stats(r.evidence_calc_with_remaining[3], r.evidence_calc_with_remaining[5], &mu1, &v1);
+    linear_to_log_stats(r.evidence_calc[3], r.evidence_calc[5], &mu0, &v0);
+    set((mu1 - mu0) < c.dlogZ, 2);
+  }
+  if (c.has_ess) {
+    double ess = exp((r.evidence_calc_with_remaining[3] + r.evidence_calc_with_remaining[3]) -
+                     r.evidence_calc_with_remaining[7]);
+    set(ess >= c.ess, 3);
+  }
+  if (c.has_max_num_likelihood_evaluations)
+    set((double)r.num_likelihood_evaluations >= c.max_num_likelihood_evaluations, 4);
+  if (c.has_log_L_contour) set(r.log_L_contour >= c.log_L_contour, 5);
+  if (c.has_efficiency_threshold) set(r.efficiency < c.efficiency_threshold, 6);
+  set(r.plateau != 0, 7);
+  if (c.has_rtol) set(r.relative_spread < c.rtol, 8);
+  if (c.has_atol) set(r.absolute_spread < c.atol, 9);
+  set(r.no_seed_points != 0, 10);
+  if (c.has_peak_XL_frac) {
+    double log_XL = r.evidence_calc[1] + r.evidence_calc[0];
+    set(log_XL < r.peak_log_XL + log(c.peak_XL_frac), 11);
+  }
+  *reason = why;
+  return done;
+}
+
+// key order of a stable FP64 sort: (key, index) lexicographic with NaN last
+__device__ __forceinline__ bool key_lt(double ka, int ia, double kb, int ib) {
+  bool na = __builtin_isnan(ka), nb = __builtin_isnan(kb);
+  if (na || nb) {
+    if (na && nb) return ia < ib;
+    return nb;  // a < NaN
+  }
+  return (ka < kb) || (ka == kb && ia < ib);
+}
+// strict key order ignoring indices (NaN last)
+__device__ __forceinline__ bool kless(double a, double b) {
+  if (__builtin_isnan(a)) return false;
+  if (__builtin_isnan(b)) return true;
+  return a < b;
+}
+
+}  // namespace hipns

@@ -1,0 +1,61 @@
+// ns_sampler.h -- launch interface of the fused sampling kernel (ns_sampler.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hipns.h"
+
+namespace hipns {
+
+enum SampleMode : int {
+  MODE_SLICE = 0,  // UniDimSliceSampler chains of one shell (Philox draws)
+  MODE_UNIF = 1,   // UniformSampler rejection chains of one shell
+  MODE_INIT = 2,   // initial live points: uniform until logL > -inf
+  MODE_INJ = 3,    // slice chains with injected draws and a recorded trace
+};
+
+struct SampleArgs {
+  const hipns_run_desc_t* runs;
+  hipns_run_state_t* states;
+  hipns_buffers_t B;
+  int32_t max_steps;
+  int32_t gmm_K;
+  // injected-draw mode
+  int32_t inj_buf;
+  int32_t inj_C;
+  double inj_contour;
+  const int32_t* inj_seed;
+  const float* inj_dirs;
+  const float* inj_tu;
+  float* trace_t;
+  double* trace_logL;
+  int32_t* trace_steps;
+  float* ph_U;
+  double* ph_logL;
+};
+
+// floats of dynamic LDS one sampler block needs (layout of load_params in ns_sampler_impl.h)
+inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc) {
+  if (fam == HIPNS_FAM_MVN) {
+    int CW = 64 / lpc;
+    int MB = (D + CW - 1) / CW;
+    int RS = MB * CW + 1;
+    return (size_t)2 * dps + (size_t)dps * RS;
+  }
+  return (size_t)(3 + K) * dps + 2 * K;
+}
+
+hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
+                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s);
+
+}  // namespace hipns
+
+namespace hipns {
+hipError_t launch_sampler_slice(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dpmax,
+                                size_t lds_bytes, hipStream_t s);
+hipError_t launch_sampler_unif(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dpmax,
+                               size_t lds_bytes, hipStream_t s);
+hipError_t launch_sampler_init(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dpmax,
+                               size_t lds_bytes, hipStream_t s);
+hipError_t launch_sampler_inj(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dpmax,
+                              size_t lds_bytes, hipStream_t s);
+}  // namespace hipns

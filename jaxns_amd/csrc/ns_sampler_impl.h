@@ -1,0 +1,504 @@
+// ns_sampler_impl.h -- the fused sampling kernel of libhipns (gfx950).
+//
+// One wavefront advances CW = 64/LPC Markov chains in lock-step "evaluation
+// steps".  Lane l serves chain (l % CW) and part h = l / CW of that chain's
+// U-space vector: it owns the dimensions d(h, j) = 8*LPC*(j/8) + 8h + (j%8),
+// j < DP/LPC.  Each step every live chain proposes one point, the wavefront
+// evaluates all CW likelihoods at once (for the Gaussian family as an MFMA
+// GEMM whose B operand is exactly this layout), and each chain then accepts,
+// shrinks or starts its next slice.  Chains never wait for each other except
+// at the very end of the launch.
+//
+// Reference semantics: samplers/uni_slice_sampler.py:23-441 (slice chain),
+// samplers/uniform_samplers.py:42-85 (rejection sampler),
+// nested_samplers/common/uniform_sample.py:12-61 (initial live points),
+// samplers/bases.py:63-74 + internals/random.py:34-84 (seed choice).
+#pragma once
+#include "hipns_dev.h"
+#include "ns_sampler.h"
+
+namespace hipns {
+
+// ---------------------------------------------------------------------------
+// slice arithmetic, every operation one IEEE rounding (matches the oracle)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bound_candidates(float u0, float n, float& rmin, float& lmax) {
+#pragma clang fp contract(off)
+  float t1 = (1.0f - u0) / n;
+  float t0 = -u0 / n;
+  float c1r = t1 >= 0.0f ? t1 : __builtin_inff();
+  float c1l = t1 <= 0.0f ? t1 : -__builtin_inff();
+  float c0r = t0 >= 0.0f ? t0 : __builtin_inff();
+  float c0l = t0 <= 0.0f ? t0 : -__builtin_inff();
+  // min over {t0 right, t1 right}, max over {t0 left, t1 left}; fminf/fmaxf are exact
+  rmin = fminf(rmin, fminf(c0r, c1r));
+  lmax = fmaxf(lmax, fmaxf(c0l, c1l));
+}
+
+__device__ __forceinline__ float pick_t(float left, float right, float u) {
+#pragma clang fp contract(off)
+  return left + u * (right - left);
+}
+
+__device__ __forceinline__ float point_on_line(float u0, float t, float n) {
+#pragma clang fp contract(off)
+  return u0 + t * n;
+}
+
+__device__ __forceinline__ void shrink(float t, float alpha, bool midpoint, float& left, float& right) {
+#pragma clang fp contract(off)
+  if (t < 0.0f) left = t;
+  if (t > 0.0f) right = t;
+  if (midpoint) {
+    if (t < 0.0f) left = alpha * left;
+    if (t > 0.0f) right = alpha * right;
+  }
+}
+
+// owned dimension of lane-part h at register slot j
+template <int LPC>
+__device__ __forceinline__ int owned_dim(int h, int j) {
+  return 8 * LPC * (j >> 3) + 8 * h + (j & 7);
+}
+
+// ---------------------------------------------------------------------------
+// likelihood families in the chain-tile layout.  Parameters live in LDS.
+// ---------------------------------------------------------------------------
+struct MvnLds {
+  float* WT;  // [DP][RS]: WT[d*RS + m] = W[m][d]
+  float* b;   // [DP]
+  float* tf;  // [DP]: 1 -> ndtri, 0 -> identity
+  int RS;
+};
+
+struct GmmLds {
+  float* tf;     // [DP]
+  float* p;      // [DP]
+  float* q;      // [DP]
+  float* mean;   // [K][DP]
+  float* inv2s2; // [K]
+  float* logc;   // [K]
+  int K;
+  int dps;
+};
+
+// r = W q(u) + b ; logL = c - |r|^2 / 2 ; W rows (outputs) tiled by 32 (LPC=2) or 16 (LPC=4)
+template <int LPC, int NJ>
+__device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, int dps, const MvnLds& P,
+                                          double log_const) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  const int l = lane_id();
+  const int m = l % CW;
+  const int h = l / CW;
+  float z[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    int d = owned_dim<LPC>(h, j);
+    float v = 0.0f;
+    if (j < nj) {
+      float u = up[j];
+      v = (P.tf[d] != 0.0f) ? ndtri_f32(u) : u;
+      if (d >= D) v = 0.0f;
+    }
+    z[j] = v;
+  }
+  float ss = 0.0f;
+  const int MB = (D + CW - 1) / CW;
+  for (int mb = 0; mb < MB; ++mb) {
+    const int row = mb * CW + m;
+    if constexpr (LPC == 2) {
+      f32x16 acc0 = {0}, acc1 = {0};
+#pragma unroll
+      for (int j = 0; j < NJ; j += 2) {
+        if (j < nj) {
+          float a0 = P.WT[owned_dim<LPC>(h, j) * P.RS + row];
+          acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, z[j], acc0, 0, 0, 0);
+          float a1 = P.WT[owned_dim<LPC>(h, j + 1) * P.RS + row];
+          acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, z[j + 1], acc1, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        int r = mb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+        float v = acc0[i] + acc1[i] + P.b[r];
+        ss = fmaf(v, v, ss);
+      }
+    } else {
+      f32x4 acc0 = {0}, acc1 = {0};
+#pragma unroll
+      for (int j = 0; j < NJ; j += 2) {
+        if (j < nj) {
+          float a0 = P.WT[owned_dim<LPC>(h, j) * P.RS + row];
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, z[j], acc0, 0, 0, 0);
+          float a1 = P.WT[owned_dim<LPC>(h, j + 1) * P.RS + row];
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, z[j + 1], acc1, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int r = mb * 16 + 4 * h + i;
+        float v = acc0[i] + acc1[i] + P.b[r];
+        ss = fmaf(v, v, ss);
+      }
+    }
+  }
+  (void)dps;
+  ss = chain_sum<LPC>(ss);
+  double L = log_const - 0.5 * (double)ss;
+  return __builtin_isnan(L) ? -__builtin_inf() : L;
+}
+
+template <int LPC, int NJ>
+__device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, const GmmLds& P) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  const int h = lane_id() / CW;
+  float part[HIPNS_MAX_GMM_K];
+#pragma unroll
+  for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) part[k] = 0.0f;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    if (j < nj) {
+      int d = owned_dim<LPC>(h, j);
+      if (d < D) {
+        float u = up[j];
+        float qv = (P.tf[d] != 0.0f) ? ndtri_f32(u) : u;
+        float x = fmaf(P.q[d], qv, P.p[d]);
+#pragma unroll
+        for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
+          if (k < P.K) {
+            float r = x - P.mean[k * P.dps + d];
+            part[k] = fmaf(r, r, part[k]);
+          }
+        }
+      }
+    }
+  }
+  double acc = -__builtin_inf();
+#pragma unroll
+  for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
+    if (k < P.K) {
+      float s = chain_sum<LPC>(part[k]);
+      double lk = (double)P.logc[k] - (double)s * (double)P.inv2s2[k];
+      acc = lae(acc, lk);
+    }
+  }
+  return __builtin_isnan(acc) ? -__builtin_inf() : acc;
+}
+
+// ---------------------------------------------------------------------------
+// the sampling kernel
+// ---------------------------------------------------------------------------
+
+
+// dynamic LDS layout of one block (one run per block)
+__device__ __forceinline__ void load_params(int fam, const float* prm, int dps, int D, int K, float* lds, MvnLds& mv,
+                                            GmmLds& gm, int LPCv) {
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  if (fam == HIPNS_FAM_MVN) {
+    const int CW = HIPNS_WAVE / LPCv;
+    const int MB = (D + CW - 1) / CW;
+    const int RS = MB * CW + 1;
+    mv.tf = lds;
+    mv.b = lds + dps;
+    mv.WT = lds + 2 * dps;
+    mv.RS = RS;
+    for (int i = tid; i < dps; i += nt) {
+      mv.tf[i] = prm[i];
+    }
+    for (int i = tid; i < dps; i += nt) mv.b[i] = prm[dps + i];
+    const float* W = prm + 2 * dps;
+    for (int i = tid; i < dps * RS; i += nt) {
+      int d = i / RS, m = i % RS;
+      mv.WT[i] = (m < dps && m < MB * CW) ? W[m * dps + d] : 0.0f;
+    }
+  } else {
+    gm.tf = lds;
+    gm.p = lds + dps;
+    gm.q = lds + 2 * dps;
+    gm.mean = lds + 3 * dps;
+    gm.inv2s2 = lds + 3 * dps + K * dps;
+    gm.logc = gm.inv2s2 + K;
+    gm.K = K;
+    gm.dps = dps;
+    for (int i = tid; i < (3 + K) * dps + 2 * K; i += nt) lds[i] = prm[i];
+  }
+  __syncthreads();
+}
+
+
+template <int LPC, int NJ, int FAM, int MODE>
+__global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  extern __shared__ float lds[];
+  const int r = blockIdx.y;
+  const hipns_run_desc_t run = A.runs[r];
+  hipns_run_state_t* st = A.states + r;
+  if (MODE != MODE_INJ && st->done) return;  // uniform across the block
+  const int wpb = blockDim.x / HIPNS_WAVE;
+  const int wave = threadIdx.x / HIPNS_WAVE;
+  const int l = lane_id();
+  const int c = l % CW;
+  const int h = l / CW;
+  const int nchains = (MODE == MODE_INIT) ? run.N : ((MODE == MODE_INJ) ? A.inj_C : run.shell);
+  const int chain0 = (blockIdx.x * wpb + wave) * CW;
+  if (blockIdx.x * wpb * CW >= nchains) return;  // whole block idle
+  const int dps = A.B.dps;
+  const int D = run.D;
+  const int DPr = ((D + 8 * LPC - 1) / (8 * LPC)) * (8 * LPC);
+  const int nj = DPr / LPC;
+
+  MvnLds mv;
+  GmmLds gm;
+  load_params(FAM, A.B.params + run.param_off, dps, D, A.gmm_K, lds, mv, gm, LPC);
+
+  const int chain = chain0 + c;
+  bool alive = chain < nchains;
+  const int64_t iter = (MODE == MODE_INIT) ? 0xFFFFFFFFll : ((MODE == MODE_INJ) ? 0 : st->iteration);
+  const uint32_t k0 = run.seed_lo, k1 = run.seed_hi;
+  const uint32_t it32 = (uint32_t)iter;
+  const int cur = (MODE == MODE_INJ) ? A.inj_buf : st->buf;
+  const double contour = (MODE == MODE_INJ) ? A.inj_contour : ((MODE == MODE_INIT) ? -__builtin_inf() : st->contour);
+  const int N = run.N;
+  const int S = run.shell;
+  const int num_slices = run.num_slices;
+  const int kph = run.k_phantom;
+  const bool midpoint = run.midpoint_shrink != 0;
+
+  float U0[NJ], nv[NJ], up[NJ];
+  double logL0 = -__builtin_inf();
+  int64_t evals = 0;
+  int s = 0, k = 0;
+  float left = 0.0f, right = 0.0f, t = 0.0f, alpha = 0.5f;
+  int cap_hits = 0;
+
+  // ---- seed point (get_seed_point, uni_slice_sampler.py:343-358)
+  if (MODE == MODE_SLICE || MODE == MODE_INJ) {
+    int idx = 0;
+    if (alive) {
+      if (MODE == MODE_INJ) {
+        idx = A.inj_seed[chain];
+      } else {
+        u32x4 w = philox4x32((uint32_t)chain, it32, 0xFFFFFFFFu, RNG_SEED, k0, k1);
+        double u = u01d(w.x, w.y);
+        int a = st->first_seed;
+        int M = N - a;
+        // first i >= a with (i - a + 1) >= M (1 - u): i = a + ceil(M(1-u)) - 1 (random.py:57-60)
+        idx = M > 0 ? a + (int)ceil((double)M * (1.0 - u)) - 1 : 0;
+        idx = idx < 0 ? 0 : (idx > N - 1 ? N - 1 : idx);
+      }
+    }
+    const float* src = A.B.live_U[cur] + (run.live_off + idx) * (int64_t)dps;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      int d = owned_dim<LPC>(h, j);
+      U0[j] = (alive && j < nj) ? src[d] : 0.5f;
+    }
+    logL0 = alive ? A.B.live_logL[cur][run.live_off + idx] : 0.0;
+  } else {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) U0[j] = 0.5f;
+  }
+
+  // ---- begin a slice: direction, perfect bracket, first pick
+  auto begin_slice = [&](int si) {
+    k = 0;
+    alpha = (float)(num_slices > 1 ? 0.5 + 0.5 * (double)si / (double)(num_slices - 1) : 0.5);
+    if (MODE == MODE_INJ) {
+      const float* dsrc = A.inj_dirs + ((int64_t)chain * num_slices + si) * dps;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) nv[j] = (j < nj) ? dsrc[owned_dim<LPC>(h, j)] : 0.0f;
+    } else if (D == 1) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) nv[j] = (owned_dim<LPC>(h, j) == 0) ? 1.0f : 0.0f;
+    } else {
+      // _sample_direction (uni_slice_sampler.py:23-38): z ~ N(0, I), n = z / |z|
+      float ss = 0.0f;
+#pragma unroll
+      for (int j = 0; j < NJ; j += 4) {
+        if (j < nj) {
+          int d0 = owned_dim<LPC>(h, j);
+          u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)si, RNG_DIR | (uint32_t)(d0 >> 2), k0, k1);
+          float r0 = sqrtf(-2.0f * __logf(u01f_open0(w.x)));
+          float r1 = sqrtf(-2.0f * __logf(u01f_open0(w.z)));
+          float s0, c0, s1, c1;
+          __sincosf(6.28318530717958648f * u01f(w.y), &s0, &c0);
+          __sincosf(6.28318530717958648f * u01f(w.w), &s1, &c1);
+          float zz[4] = {r0 * c0, r0 * s0, r1 * c1, r1 * s1};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            int d = d0 + q;
+            float v = d < D ? zz[q] : 0.0f;
+            nv[j + q] = v;
+            ss = fmaf(v, v, ss);
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) nv[j + q] = 0.0f;
+        }
+      }
+      ss = chain_sum<LPC>(ss);
+      float nrm = sqrtf(ss);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) nv[j] = nv[j] / nrm;
+    }
+    // _slice_bounds (uni_slice_sampler.py:41-64)
+    float rmin = __builtin_inff(), lmax = -__builtin_inff();
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (j < nj && owned_dim<LPC>(h, j) < D) bound_candidates(U0[j], nv[j], rmin, lmax);
+    right = chain_min<LPC>(rmin);
+    left = chain_max<LPC>(lmax);
+    float u;
+    if (MODE == MODE_INJ) {
+      u = A.inj_tu[((int64_t)chain * num_slices + si) * A.max_steps + 0];
+    } else {
+      u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)si, RNG_PICK, k0, k1);
+      u = u01f(w.x);
+    }
+    t = pick_t(left, right, u);
+  };
+
+  auto draw_uniform_point = [&](int attempt) {
+#pragma unroll
+    for (int j = 0; j < NJ; j += 4) {
+      int d0 = owned_dim<LPC>(h, j);
+      u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)attempt, RNG_UNIF | (uint32_t)(d0 >> 2), k0, k1);
+      float uu[4] = {u01f(w.x), u01f(w.y), u01f(w.z), u01f(w.w)};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) up[j + q] = (j < nj && d0 + q < D) ? uu[q] : 0.5f;
+    }
+  };
+
+  bool need_begin = (MODE == MODE_SLICE || MODE == MODE_INJ) && alive;
+
+  const int64_t phantom_base = (MODE == MODE_SLICE) ? (st->next_idx + S) % run.max_samples : 0;
+  const int sender = (MODE == MODE_SLICE) ? st->sender_node : 0;
+  const int max_uniform = (MODE == MODE_INIT) ? 1000000 : run.max_uniform_evals;
+
+  while (__any(alive)) {
+    if (MODE == MODE_SLICE || MODE == MODE_INJ) {
+      if (need_begin) {
+        begin_slice(s);
+        need_begin = false;
+      }
+    }
+    // ---- proposal
+    if (MODE == MODE_SLICE || MODE == MODE_INJ) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) up[j] = point_on_line(U0[j], t, nv[j]);
+    } else {
+      draw_uniform_point((int)evals);
+    }
+    // ---- batched likelihood of the wavefront's CW proposals (Model.forward)
+    double L;
+    if (FAM == HIPNS_FAM_MVN)
+      L = lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const);
+    else
+      L = lik_gmm<LPC, NJ>(up, nj, D, gm);
+
+    if (alive) {
+      ++evals;
+      if (MODE == MODE_INJ) {
+        int64_t o = ((int64_t)chain * num_slices + s) * A.max_steps + k;
+        if (h == 0) {
+          A.trace_t[o] = t;
+          A.trace_logL[o] = L;
+        }
+      }
+      if (MODE == MODE_INIT || MODE == MODE_UNIF) {
+        // uniform_sample.py:35-36 (logL > -inf) / uniform_samplers.py:44-50
+        bool acc = (MODE == MODE_INIT) ? (L > -__builtin_inf()) : ((L > contour) || (L == contour));
+        if (!acc && evals >= max_uniform) acc = true;
+        if (acc) {
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) U0[j] = up[j];
+          logL0 = L;
+          alive = false;
+          const int64_t o = run.chain_off + chain;
+          float* dst = A.B.new_U + o * dps;
+#pragma unroll
+          for (int j = 0; j < NJ; ++j)
+            if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+          if (h == 0) {
+            A.B.new_logL[o] = logL0;
+            A.B.new_evals[o] = (int32_t)evals;
+          }
+        }
+      } else {
+        // _new_proposal acceptance (uni_slice_sampler.py:160-166)
+        bool acc = (L > contour) || (logL0 == contour && L == contour);
+        if (!acc && k + 1 >= A.max_steps) {
+          // step cap: keep the seed point (t = 0 lies inside every bracket)
+          acc = true;
+          L = logL0;
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) up[j] = U0[j];
+          ++cap_hits;
+        }
+        if (acc) {
+          if (MODE == MODE_INJ && h == 0) A.trace_steps[(int64_t)chain * num_slices + s] = k + 1;
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) U0[j] = up[j];
+          logL0 = L;
+          // phantom samples: x[-(k+1):-1] (uni_slice_sampler.py:432-440), evals 0
+          int pidx = s - (num_slices - 1 - kph);
+          if (kph > 0 && pidx >= 0 && s < num_slices - 1) {
+            if (MODE == MODE_INJ) {
+              float* pd = A.ph_U + ((int64_t)chain * kph + pidx) * dps;
+#pragma unroll
+              for (int j = 0; j < NJ; ++j)
+                if (j < nj) pd[owned_dim<LPC>(h, j)] = U0[j];
+              if (h == 0) A.ph_logL[(int64_t)chain * kph + pidx] = L;
+            } else {
+              int64_t si = (phantom_base + (int64_t)chain * kph + pidx) % run.max_samples + run.samp_off;
+              float* pd = A.B.samp_U + si * dps;
+#pragma unroll
+              for (int j = 0; j < NJ; ++j)
+                if (j < nj) pd[owned_dim<LPC>(h, j)] = U0[j];
+              if (h == 0) {
+                A.B.samp_logL[si] = L;
+                A.B.samp_sender[si] = sender;
+                A.B.samp_evals[si] = 0;
+                A.B.samp_phantom[si] = 1;
+              }
+            }
+          }
+          ++s;
+          if (s == num_slices) {
+            alive = false;
+            const int64_t o = run.chain_off + chain;
+            float* dst = A.B.new_U + o * dps;
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+              if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+            if (h == 0) {
+              A.B.new_logL[o] = logL0;
+              A.B.new_evals[o] = (int32_t)evals;
+            }
+          } else {
+            need_begin = true;
+          }
+        } else {
+          shrink(t, alpha, midpoint, left, right);
+          ++k;
+          float u;
+          if (MODE == MODE_INJ) {
+            u = A.inj_tu[((int64_t)chain * num_slices + s) * A.max_steps + k];
+          } else {
+            u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)s, RNG_PICK | (uint32_t)(k >> 2), k0, k1);
+            const int kw = k & 3;
+            u = u01f(kw == 0 ? w.x : (kw == 1 ? w.y : (kw == 2 ? w.z : w.w)));
+          }
+          t = pick_t(left, right, u);
+        }
+      }
+    }
+  }
+  if (cap_hits > 0 && h == 0 && MODE != MODE_INJ) {
+    atomicAdd((unsigned long long*)&st->cap_hits, (unsigned long long)cap_hits);
+    atomicOr(&st->flags, 1);
+  }
+}
+
+}  // namespace hipns

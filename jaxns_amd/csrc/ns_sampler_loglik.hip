@@ -1,0 +1,68 @@
+// ns_sampler_loglik.hip -- batched likelihood through the sampler's device code.
+#include "ns_sampler_impl.h"
+
+namespace hipns {
+
+// ---------------------------------------------------------------------------
+// batched likelihood (Model.forward over a batch) through the same device code:
+// row i of U plays the role of chain i of a single wavefront tile.
+// ---------------------------------------------------------------------------
+template <int LPC, int NJ, int FAM>
+__global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
+                                                hipns_buffers_t Bf, int K) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  extern __shared__ float lds[];
+  const hipns_run_desc_t run = runs[0];
+  const int dps = Bf.dps;
+  const int D = run.D;
+  const int DPr = ((D + 8 * LPC - 1) / (8 * LPC)) * (8 * LPC);
+  const int nj = DPr / LPC;
+  MvnLds mv;
+  GmmLds gm;
+  load_params(FAM, Bf.params + run.param_off, dps, D, K, lds, mv, gm, LPC);
+  const int wpb = blockDim.x / HIPNS_WAVE;
+  const int wave = threadIdx.x / HIPNS_WAVE;
+  const int l = lane_id();
+  const int c = l % CW, h = l / CW;
+  const int64_t row = ((int64_t)blockIdx.x * wpb + wave) * CW + c;
+  float up[NJ];
+  const bool ok = row < B;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    int d = owned_dim<LPC>(h, j);
+    up[j] = (ok && j < nj && d < D) ? U[row * dps + d] : 0.5f;
+  }
+  double L = (FAM == HIPNS_FAM_MVN) ? lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const)
+                                    : lik_gmm<LPC, NJ>(up, nj, D, gm);
+  if (ok && h == 0) out[row] = L;
+}
+
+template <int LPC, int FAM>
+static hipError_t launch_loglik_dp(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
+                                   const hipns_buffers_t& Bf, int K, int wpb, int dpmax, size_t lds, hipStream_t s) {
+  if (LPC == 2 && dpmax > 64) return launch_loglik_dp<4, FAM>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s);
+  constexpr int CW = HIPNS_WAVE / LPC;
+  int64_t groups = (B + CW - 1) / CW;
+  dim3 grid((unsigned)((groups + wpb - 1) / wpb)), block(wpb * HIPNS_WAVE);
+  if (dpmax <= 32)
+    hipLaunchKernelGGL((k_loglik<LPC, 32 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K);
+  else if (dpmax <= 64)
+    hipLaunchKernelGGL((k_loglik<LPC, 64 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K);
+  else if constexpr (LPC == 4)
+    hipLaunchKernelGGL((k_loglik<LPC, 128 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K);
+  return hipGetLastError();
+}
+
+hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
+                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s) {
+  if (L.family == HIPNS_FAM_GMM) {
+    if (L.lanes_per_chain == 4)
+      return launch_loglik_dp<4, HIPNS_FAM_GMM>(U, B, out, runs, Bf, L.gmm_K, L.waves_per_block, dpmax, lds, s);
+    return launch_loglik_dp<2, HIPNS_FAM_GMM>(U, B, out, runs, Bf, L.gmm_K, L.waves_per_block, dpmax, lds, s);
+  }
+  if (L.lanes_per_chain == 4)
+    return launch_loglik_dp<4, HIPNS_FAM_MVN>(U, B, out, runs, Bf, L.gmm_K, L.waves_per_block, dpmax, lds, s);
+  return launch_loglik_dp<2, HIPNS_FAM_MVN>(U, B, out, runs, Bf, L.gmm_K, L.waves_per_block, dpmax, lds, s);
+}
+
+}  // namespace hipns

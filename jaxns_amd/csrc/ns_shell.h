@@ -1,0 +1,33 @@
+// ns_shell.h -- kernels of the shell loop (ns_shell.hip) and the to_results
+// primitives (ns_results.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hipns.h"
+
+namespace hipns {
+
+__global__ void k_discard(const hipns_run_desc_t* runs, hipns_run_state_t* states, hipns_buffers_t B);
+__global__ void k_replace(const hipns_run_desc_t* runs, hipns_run_state_t* states, hipns_buffers_t B, int init_mode);
+__global__ void k_register(const hipns_run_desc_t* runs, hipns_run_state_t* states, hipns_buffers_t B,
+                           hipns_term_cond_t term);
+__global__ void k_check_term(const hipns_run_desc_t* runs, hipns_run_state_t* states, int R, hipns_term_cond_t term,
+                             int set_no_seed, hipns_buffers_t B);
+__global__ void k_init_state(hipns_run_state_t* states, int R);
+__global__ void k_init_samples(const hipns_run_desc_t* runs, int R, hipns_buffers_t B);
+__global__ void k_consume(const hipns_run_desc_t* runs, hipns_run_state_t* states, hipns_buffers_t B);
+__global__ void k_consume_advance(const hipns_run_desc_t* runs, hipns_run_state_t* states, int R);
+
+size_t replace_lds_bytes(int N);
+
+// to_results primitives (ns_results.hip)
+hipError_t evidence_scan(const double* logL, const double* n, int64_t M, int64_t stop, double* per8, double* fin8,
+                         void* ws, size_t ws_bytes, hipStream_t s);
+size_t evidence_scan_ws(int64_t M);
+hipError_t argsort_stable(const double* keys, int64_t n, int32_t* perm, void* ws, size_t ws_bytes, hipStream_t s);
+size_t argsort_ws(int64_t n);
+hipError_t count_crossed_edges(const int32_t* sender, const double* logL, int64_t N, int64_t num_samples,
+                               int64_t* samples_indices, int64_t* num_live, void* ws, size_t ws_bytes, hipStream_t s);
+size_t count_crossed_edges_ws(int64_t N);
+
+}  // namespace hipns

@@ -1,0 +1,102 @@
+"""The distribution set the prior transform supports, with TFP's names.
+
+Each distribution exposes its quantile map in the affine form the device
+kernels evaluate: X = loc + S q(U), q = ndtri (Gaussian) or identity
+(uniform).  ``forward`` is TFP's ``quantile`` followed by bijectors
+(framework/wrapped_tfp_distribution.py:77-84).
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.special import ndtri
+
+Q_IDENTITY = 0
+Q_NDTRI = 1
+
+
+class Distribution:
+    num_dims: int
+
+    def affine_form(self):
+        raise NotImplementedError
+
+    def forward(self, U):
+        k, loc, S = self.affine_form()
+        U = np.asarray(U, dtype=np.float64)
+        q = np.where(np.asarray(k) == Q_NDTRI, ndtri(U), U)
+        return loc + q @ S.T
+
+    def log_prob_from_u(self, U):
+        raise NotImplementedError
+
+
+def _vec(a, n=None):
+    a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+    if n is not None and a.size == 1 and n > 1:
+        a = np.full(n, float(a[0]))
+    return a
+
+
+class Normal(Distribution):
+    """tfpd.Normal(loc, scale): x = loc + scale * ndtri(u)."""
+
+    def __init__(self, loc=0.0, scale=1.0):
+        loc = np.atleast_1d(np.asarray(loc, dtype=np.float64))
+        scale = np.atleast_1d(np.asarray(scale, dtype=np.float64))
+        n = max(loc.size, scale.size)
+        self.loc = _vec(loc, n)
+        self.scale = _vec(scale, n)
+        self.num_dims = n
+
+    def affine_form(self):
+        return np.full(self.num_dims, Q_NDTRI, np.int32), self.loc, np.diag(self.scale)
+
+    def log_prob_from_u(self, U):
+        z = ndtri(np.asarray(U, dtype=np.float64))
+        return np.sum(-0.5 * z * z - 0.5 * np.log(2 * np.pi) - np.log(self.scale), axis=-1)
+
+    def __repr__(self):
+        return f"Normal(loc={self.loc}, scale={self.scale})"
+
+
+class Uniform(Distribution):
+    """tfpd.Uniform(low, high): x = low + (high - low) u."""
+
+    def __init__(self, low=0.0, high=1.0):
+        low = np.atleast_1d(np.asarray(low, dtype=np.float64))
+        high = np.atleast_1d(np.asarray(high, dtype=np.float64))
+        n = max(low.size, high.size)
+        self.low = _vec(low, n)
+        self.high = _vec(high, n)
+        self.num_dims = n
+
+    def affine_form(self):
+        return np.full(self.num_dims, Q_IDENTITY, np.int32), self.low, np.diag(self.high - self.low)
+
+    def log_prob_from_u(self, U):
+        U = np.asarray(U, dtype=np.float64)
+        return np.full(U.shape[:-1], -np.sum(np.log(self.high - self.low)))
+
+    def __repr__(self):
+        return f"Uniform(low={self.low}, high={self.high})"
+
+
+class MultivariateNormalTriL(Distribution):
+    """tfpd.MultivariateNormalTriL(loc, scale_tril): x = loc + L ndtri(u)."""
+
+    def __init__(self, loc, scale_tril):
+        self.loc = np.atleast_1d(np.asarray(loc, dtype=np.float64))
+        self.scale_tril = np.tril(np.asarray(scale_tril, dtype=np.float64))
+        self.num_dims = self.loc.size
+        if self.scale_tril.shape != (self.num_dims, self.num_dims):
+            raise ValueError("scale_tril must be [D, D]")
+
+    def affine_form(self):
+        return np.full(self.num_dims, Q_NDTRI, np.int32), self.loc, self.scale_tril
+
+    def log_prob_from_u(self, U):
+        z = ndtri(np.asarray(U, dtype=np.float64))
+        return np.sum(-0.5 * z * z - 0.5 * np.log(2 * np.pi), axis=-1) - np.sum(np.log(np.diag(self.scale_tril)))
+
+    def __repr__(self):
+        return f"MultivariateNormalTriL(loc={self.loc})"

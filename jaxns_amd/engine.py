@@ -1,0 +1,370 @@
+"""Device engine: a batch of nested-sampling runs advanced by libhipns kernels.
+
+Layout in HBM (one allocation per field, runs packed back to back):
+  live set   (double-buffered)  U [sum N, dps] f32 | logL, constraint f64 | sender, evals i32
+  samples    U [sum max_samples, dps] f32 | logL f64 | sender, evals i32 | phantom u8
+  new chains U [sum N, dps] f32 | logL f64 | evals i32
+  params     packed float32 per-run model parameters
+  runs       hipns_run_desc_t [R] (read-only)      states hipns_run_state_t [R]
+``dps`` is the padded U-space stride (a multiple of 32).  Torch only provides
+the device memory and the stream; every computation is a libhipns kernel.
+
+The host loop mirrors ShardedStaticNestedSampler._run
+(nested_samplers/sharded/sharded_static.py:775-851): create_init_state, the
+uniform phase (whose termination check runs first and stops it before any
+shell when efficiency_threshold > 0, as in the reference), the slice phase
+until determine_termination, then the final live points are consumed.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import Buffers, Launch, RunDesc, RunState, check, lib, ptr
+from .types import (NestedSamplerResults, TerminationCondition, TerminationConditionConjunction,
+                    TerminationConditionDisjunction)
+
+INT64_MAX = float(np.iinfo(np.int64).max)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def round_up_num_live_points(init_num_live_points, shell_frac, num_devices):
+    """sharded_static.py:577-584."""
+    n = int(init_num_live_points)
+    while True:
+        shell = int(n * shell_frac)
+        if shell % num_devices == 0:
+            return n
+        n += 1
+
+
+def round_up_max_samples(init_max_samples, num_discard, num_phantom_points):
+    """sharded_static.py:587-594."""
+    m = int(init_max_samples)
+    block = num_discard * (1 + num_phantom_points)
+    while m % block != 0:
+        m += 1
+    return m
+
+
+@dataclass
+class RunSpec:
+    model: object
+    num_live_points: int
+    num_slices: int
+    k_phantom: int = 0
+    max_samples: int = 0
+    seed: int = 0
+    midpoint_shrink: bool = True
+    shell_fraction: float = 0.5
+
+    def __post_init__(self):
+        self.shell_fraction = max(self.shell_fraction, 1.0 / self.num_live_points)
+        self.num_live_points = round_up_num_live_points(self.num_live_points, self.shell_fraction, 1)
+        self.shell = int(self.num_live_points * self.shell_fraction)
+        self.max_samples = round_up_max_samples(self.max_samples, self.shell, self.k_phantom)
+
+
+def flatten_term_cond(tc) -> dict:
+    if isinstance(tc, (TerminationConditionConjunction, TerminationConditionDisjunction)):
+        raise NotImplementedError("compound termination conditions are evaluated per flat condition only")
+    d = {}
+    for k in _lib._TC_NAMES:
+        v = getattr(tc, k)
+        if v is not None:
+            d[k] = float(v)
+    return d
+
+
+def seed_words(key) -> tuple:
+    a = np.asarray(key).ravel()
+    if a.size >= 2:
+        return int(a[0]) & 0xFFFFFFFF, int(a[1]) & 0xFFFFFFFF
+    v = int(a[0]) if a.size else 0
+    return v & 0xFFFFFFFF, (v >> 32) & 0xFFFFFFFF
+
+
+class DeviceBatch:
+    """R independent runs sharing one likelihood family, advanced together."""
+
+    def __init__(self, specs: List[RunSpec], *, lanes_per_chain: int = 2, waves_per_block: int = 4,
+                 max_steps: int = 512, device: str = "cuda"):
+        torch = _torch()
+        if not specs:
+            raise ValueError("no runs")
+        self.specs = specs
+        self.R = len(specs)
+        fam = specs[0].model.log_likelihood
+        self.family = getattr(fam, "family", None)
+        if self.family is None:
+            raise NotImplementedError("the device engine needs a fused likelihood family (jaxns_amd.likelihoods)")
+        for s in specs:
+            if getattr(s.model.log_likelihood, "family", None) != self.family:
+                raise ValueError("all runs of a batch must share the likelihood family")
+        self.K = getattr(fam, "K", 1) if self.family == _lib.FAM_GMM else 1
+        if self.family == _lib.FAM_GMM and any(getattr(s.model.log_likelihood, "K", 0) != self.K for s in specs):
+            raise ValueError("all mixture runs of a batch must have the same number of components")
+        maxD = max(s.model.U_ndims for s in specs)
+        if maxD > _lib.MAX_DIM:
+            raise ValueError(f"U dimension {maxD} exceeds {_lib.MAX_DIM}")
+        self.dps = int(math.ceil(maxD / 32) * 32)
+        self.device = torch.device(device)
+        dev = self.device
+        dps = self.dps
+        Ns = [s.num_live_points for s in specs]
+        MSs = [s.max_samples for s in specs]
+        sumN, sumMS = sum(Ns), sum(MSs)
+        # parameters
+        prms, consts = [], []
+        for s in specs:
+            p, c = s.model.log_likelihood.device_params(s.model, dps)
+            prms.append(p)
+            consts.append(c)
+        poffs = np.cumsum([0] + [p.size for p in prms])
+        self.params = torch.from_numpy(np.concatenate(prms)).to(dev)
+        # descriptors
+        arr = (RunDesc * self.R)()
+        lo = so = 0
+        for r, s in enumerate(specs):
+            d = arr[r]
+            d.D = s.model.U_ndims
+            d.N = s.num_live_points
+            d.shell = s.shell
+            d.num_slices = s.num_slices
+            d.k_phantom = s.k_phantom
+            d.midpoint_shrink = int(s.midpoint_shrink)
+            d.sampler = 0
+            d.max_uniform_evals = 100
+            d.max_samples = s.max_samples
+            d.live_off = lo
+            d.samp_off = so
+            d.chain_off = lo
+            d.param_off = int(poffs[r])
+            d.seed_lo, d.seed_hi = seed_words(s.seed)
+            d.log_const = consts[r]
+            lo += s.num_live_points
+            so += s.max_samples
+        self.runs_host = arr
+        self.runs_dev = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+        self.states_dev = torch.zeros(self.R * C.sizeof(RunState), dtype=torch.uint8, device=dev)
+        f32, f64, i32 = torch.float32, torch.float64, torch.int32
+        self.live_U = [torch.full((sumN, dps), 0.5, dtype=f32, device=dev) for _ in range(2)]
+        self.live_logL = [torch.zeros(sumN, dtype=f64, device=dev) for _ in range(2)]
+        self.live_cons = [torch.zeros(sumN, dtype=f64, device=dev) for _ in range(2)]
+        self.live_sender = [torch.zeros(sumN, dtype=i32, device=dev) for _ in range(2)]
+        self.live_evals = [torch.zeros(sumN, dtype=i32, device=dev) for _ in range(2)]
+        self.samp_U = torch.empty((sumMS, dps), dtype=f32, device=dev)
+        self.samp_logL = torch.empty(sumMS, dtype=f64, device=dev)
+        self.samp_sender = torch.empty(sumMS, dtype=i32, device=dev)
+        self.samp_evals = torch.empty(sumMS, dtype=i32, device=dev)
+        self.samp_phantom = torch.empty(sumMS, dtype=torch.uint8, device=dev)
+        self.new_U = torch.full((sumN, dps), 0.5, dtype=f32, device=dev)
+        self.new_logL = torch.zeros(sumN, dtype=f64, device=dev)
+        self.new_evals = torch.zeros(sumN, dtype=i32, device=dev)
+        b = Buffers()
+        for i in range(2):
+            b.live_U[i] = ptr(self.live_U[i])
+            b.live_logL[i] = ptr(self.live_logL[i])
+            b.live_cons[i] = ptr(self.live_cons[i])
+            b.live_sender[i] = ptr(self.live_sender[i])
+            b.live_evals[i] = ptr(self.live_evals[i])
+        b.samp_U, b.samp_logL, b.samp_sender = ptr(self.samp_U), ptr(self.samp_logL), ptr(self.samp_sender)
+        b.samp_evals, b.samp_phantom = ptr(self.samp_evals), ptr(self.samp_phantom)
+        b.new_U, b.new_logL, b.new_evals = ptr(self.new_U), ptr(self.new_logL), ptr(self.new_evals)
+        b.params = ptr(self.params)
+        b.dps = dps
+        self.bufs = b
+        self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, 0, max_steps)
+        self._graph = None
+        self._graph_iters = 0
+        self._graph_term = None
+
+    # ------------------------------------------------------------------ calls
+    def _s(self, stream):
+        return _lib.stream_ptr(stream)
+
+    def init_state(self, stream=None):
+        check(lib().hipns_create_init_state(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
+                                            C.byref(self.bufs), C.byref(self.launch), None, 0, self._s(stream)),
+              "hipns_create_init_state")
+
+    def check_termination(self, term: dict, set_no_seed: bool = True, stream=None):
+        tc = _lib.make_term_cond(**term)
+        check(lib().hipns_check_termination(ptr(self.runs_dev), ptr(self.states_dev), self.R, C.byref(self.bufs),
+                                            C.byref(tc), int(set_no_seed), self._s(stream)),
+              "hipns_check_termination")
+
+    def collect_shell(self, term: dict, stream=None):
+        tc = _lib.make_term_cond(**term)
+        check(lib().hipns_collect_shell(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
+                                        C.byref(self.bufs), C.byref(self.launch), C.byref(tc), None, 0,
+                                        self._s(stream)), "hipns_collect_shell")
+
+    def consume(self, stream=None):
+        check(lib().hipns_consume_live(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
+                                       C.byref(self.bufs), self._s(stream)), "hipns_consume_live")
+
+    def states(self) -> List[RunState]:
+        raw = self.states_dev.cpu().numpy().tobytes()
+        sz = C.sizeof(RunState)
+        return [RunState.from_buffer_copy(raw[i * sz:(i + 1) * sz]) for i in range(self.R)]
+
+    def done_flags(self) -> np.ndarray:
+        torch = _torch()
+        st = self.states_dev.view(self.R, C.sizeof(RunState))
+        off = RunState.done.offset
+        flags = st[:, off:off + 4].contiguous().view(torch.int32).cpu().numpy().ravel()
+        return flags
+
+    def set_sampler(self, sampler: int):
+        for r in range(self.R):
+            self.runs_host[r].sampler = sampler
+        torch = _torch()
+        self.runs_dev.copy_(torch.frombuffer(bytearray(bytes(self.runs_host)), dtype=torch.uint8))
+
+    # ------------------------------------------------------------------ loop
+    def run(self, term: dict, *, init_efficiency_threshold: float = 0.1, graph_iters: int = 0,
+            max_iterations: int = 1_000_000, stream=None):
+        """create_init_state -> phases -> consume.  graph_iters > 0 replays a captured
+        hipGraph of that many shells between host checks; 0 syncs after every shell."""
+        torch = _torch()
+        self.init_state(stream)
+        if init_efficiency_threshold > 0:
+            uterm = dict(efficiency_threshold=float(init_efficiency_threshold), dlogZ=0.0,
+                         max_samples=float(min(s.max_samples for s in self.specs)))
+            self.check_termination(uterm, True, stream)
+            if not np.all(self.done_flags()):
+                self.set_sampler(1)
+                self._loop(uterm, graph_iters=0, max_iterations=max_iterations, stream=stream)
+                self.set_sampler(0)
+        self.check_termination(term, True, stream)
+        self._loop(term, graph_iters=graph_iters, max_iterations=max_iterations, stream=stream)
+        self.consume(stream)
+        torch.cuda.synchronize()
+
+    def _loop(self, term, *, graph_iters, max_iterations, stream):
+        torch = _torch()
+        it = 0
+        if graph_iters and graph_iters > 0:
+            g = self._get_graph(term, graph_iters)
+            while not np.all(self.done_flags()) and it < max_iterations:
+                g.replay()
+                it += graph_iters
+        else:
+            while it < max_iterations:
+                if np.all(self.done_flags()):
+                    break
+                self.collect_shell(term, stream)
+                it += 1
+
+    def _get_graph(self, term, iters):
+        torch = _torch()
+        key = (tuple(sorted(term.items())), iters)
+        if self._graph is not None and self._graph_term == key:
+            return self._graph
+        s = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s, capture_error_mode="relaxed"):
+            for _ in range(iters):
+                self.collect_shell(term, stream=s)
+        self._graph = g
+        self._graph_term = key
+        return g
+
+    # ------------------------------------------------------------------ results
+    def log_Z(self, stream=None) -> np.ndarray:
+        """Per-run (log_Z_mean, log_Z_uncert) from the device tree count + evidence scan."""
+        out = []
+        for r in range(self.R):
+            res = self.results_core(r, stream=stream)
+            out.append((res["log_Z_mean"], res["log_Z_uncert"]))
+        return np.array(out)
+
+    def results_core(self, r: int, stream=None, full: bool = False) -> dict:
+        """_to_results (sharded_static.py:652-745, trim=True) for run r, on the device."""
+        torch = _torch()
+        st = self.states()[r]
+        spec = self.specs[r]
+        d = self.runs_host[r]
+        M = int(min(st.num_samples, d.max_samples))
+        off = int(d.samp_off)
+        dev = self.device
+        logL = self.samp_logL[off:off + M]
+        sender = self.samp_sender[off:off + M]
+        si = torch.empty(M, dtype=torch.int64, device=dev)
+        nl = torch.empty(M, dtype=torch.int64, device=dev)
+        ws = torch.empty(int(lib().hipns_count_crossed_edges_workspace_size(M)), dtype=torch.uint8, device=dev)
+        check(lib().hipns_count_crossed_edges(ptr(sender), ptr(logL), M, -1, ptr(si), ptr(nl), ptr(ws), ws.numel(),
+                                              self._s(stream)), "hipns_count_crossed_edges")
+        lL = logL.index_select(0, si)
+        nld = nl.to(torch.float64)
+        per8 = torch.empty((8, M), dtype=torch.float64, device=dev)
+        fin8 = torch.empty(8, dtype=torch.float64, device=dev)
+        ws2 = torch.empty(int(lib().hipns_evidence_scan_workspace_size(M)), dtype=torch.uint8, device=dev)
+        check(lib().hipns_evidence_scan(ptr(lL), ptr(nld), M, -1, ptr(per8), ptr(fin8), ptr(ws2), ws2.numel(),
+                                        self._s(stream)), "hipns_evidence_scan")
+        fin = fin8.cpu().numpy()
+        log_Z_mean, log_Z_var = _linear_to_log_stats(fin[3], fin[5])
+        ph = self.samp_phantom[off:off + M]
+        total_phantom = int(ph.to(torch.int64).sum().item())
+        pf = total_phantom / M
+        k = pf / (1.0 - pf)
+        log_Z_uncert = math.sqrt(log_Z_var) * math.sqrt(1.0 + k)
+        ess = math.exp((fin[3] + fin[3]) - fin[7]) / (1.0 + k)
+        out = dict(log_Z_mean=float(log_Z_mean), log_Z_uncert=float(log_Z_uncert), ESS=float(ess), M=M,
+                   total_phantom=total_phantom, termination_reason=int(st.termination_reason),
+                   iterations=int(st.iteration), cap_hits=int(st.cap_hits))
+        if full:
+            ev = self.samp_evals[off:off + M].index_select(0, si)
+            U = self.samp_U[off:off + M].index_select(0, si)[:, :spec.model.U_ndims]
+            out.update(samples_indices=si.cpu().numpy(), num_live_points=nl.cpu().numpy(), log_L=lL.cpu().numpy(),
+                       per8=per8.cpu().numpy(), evals=ev.to(torch.int64).cpu().numpy(),
+                       U=U.to(torch.float64).cpu().numpy())
+        return out
+
+    def to_results(self, r: int, stream=None) -> NestedSamplerResults:
+        core = self.results_core(r, stream=stream, full=True)
+        model = self.specs[r].model
+        per8 = core["per8"]
+        log_L = core["log_L"]
+        log_dZ = per8[6]
+        log_X = per8[1]
+        mx = np.max(log_dZ)
+        lnorm = mx + np.log(np.sum(np.exp(log_dZ - mx))) if np.isfinite(mx) else -np.inf
+        log_dp = np.where(np.isneginf(lnorm), -np.inf, log_dZ - lnorm)
+        p = np.exp(log_dp)
+        v = np.where(np.isneginf(log_dp), 0.0, log_L)
+        H_instable = -(np.sum(p * v) - core["log_Z_mean"])
+        H_stable = -np.sum(p * (-log_X))
+        H = H_instable if np.isfinite(H_instable) else H_stable
+        evals = core["evals"]
+        total_evals = int(np.sum(evals))
+        M = core["M"]
+        U = core["U"]
+        samples = model.transform(U)
+        log_post = log_L + model.log_prob_prior_numpy(U)
+        return NestedSamplerResults(
+            log_Z_mean=core["log_Z_mean"], log_Z_uncert=core["log_Z_uncert"], ESS=core["ESS"], H_mean=float(H),
+            samples=samples, parametrised_samples={}, U_samples=U, log_L_samples=log_L, log_dp_mean=log_dp,
+            log_X_mean=log_X, log_posterior_density=log_post, num_live_points_per_sample=core["num_live_points"],
+            num_likelihood_evaluations_per_sample=evals, total_num_samples=M,
+            total_phantom_samples=core["total_phantom"], total_num_likelihood_evaluations=total_evals,
+            log_efficiency=float(np.log(M) - np.log(total_evals)), termination_reason=core["termination_reason"])
+
+
+def _linear_to_log_stats(log_f_mean, log_f2_mean):
+    """internals/stats.py:55-74."""
+    fsq = log_f_mean + log_f_mean
+    mu = fsq - 0.5 * log_f2_mean
+    s2 = log_f2_mean - fsq
+    return mu, (s2 if np.isnan(s2) else max(s2, np.finfo(np.float64).eps))

@@ -1,0 +1,49 @@
+"""Synthetic problems of the benchmark configurations (seeded numpy, FP64 generation)."""
+import numpy as np
+from scipy.stats import multivariate_normal
+
+import jaxns_amd as J
+from jaxns_amd import distributions as tfpd
+
+
+def mvn_problem(D, seed=0):
+    """mu ~ N(0, 0.5^2 I), Sigma = A A^T / D + 0.05 I, prior N(0, I); log Z = log N(mu; 0, I + Sigma)."""
+    rng = np.random.default_rng(seed)
+    mu = rng.normal(0.0, 0.5, D)
+    A = rng.normal(size=(D, D))
+    Sigma = A @ A.T / D + 0.05 * np.eye(D)
+    log_Z = multivariate_normal(np.zeros(D), np.eye(D) + Sigma).logpdf(mu)
+
+    def prior_model():
+        x = yield J.Prior(tfpd.Normal(loc=np.zeros(D), scale=np.ones(D)), name="x")
+        return x
+
+    lik = J.MultivariateNormalLikelihood(mu, cov=Sigma)
+    return J.Model(prior_model, lik), float(log_Z), mu, Sigma
+
+
+def gmm_problem(D, seed=0, K=4):
+    """4 equal-weight isotropic components, means ~ U[-5,5]^D, sigma ~ U[0.2,1], prior U[-10,10]^D."""
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(-5, 5, size=(K, D))
+    sig = rng.uniform(0.2, 1.0, size=K)
+
+    def prior_model():
+        x = yield J.Prior(tfpd.Uniform(low=-10.0 * np.ones(D), high=10.0 * np.ones(D)), name="x")
+        return x
+
+    lik = J.GaussianMixtureLikelihood(means, sig)
+    return J.Model(prior_model, lik), lik.log_evidence_uniform_prior(-10.0, 10.0), means, sig
+
+
+def mvn_oracle_spec(D, mu, Sigma):
+    import oracle as O
+    return O.ModelSpec(O.FAM_MVN, D, O.PRIOR_NORMAL, np.zeros(D), np.ones(D), mvn_mu=mu,
+                       mvn_chol=np.linalg.cholesky(Sigma))
+
+
+def gmm_oracle_spec(D, means, sig):
+    import oracle as O
+    K = means.shape[0]
+    return O.ModelSpec(O.FAM_GMM, D, O.PRIOR_UNIFORM, -10.0 * np.ones(D), 20.0 * np.ones(D), gmm_means=means,
+                       gmm_sigma=sig, gmm_logw=np.log(np.full(K, 1.0 / K)))

@@ -1,0 +1,275 @@
+"""GPU parity of the libhipns kernels against the CPU oracle (called through the C ABI)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from jaxns_amd import _lib
+from jaxns_amd._lib import check, lib, ptr
+
+from problems import gmm_oracle_spec, gmm_problem, mvn_oracle_spec, mvn_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    return _lib.stream_ptr()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_philox_kat():
+    """Random123 known-answer vectors for Philox4x32-10."""
+    torch = _torch()
+    cases = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+             ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+             ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+              [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+    for ctr, key, want in cases:
+        c = torch.tensor(np.array(ctr, np.uint32).view(np.int32), device="cuda")
+        o = torch.empty(4, dtype=torch.int32, device="cuda")
+        check(lib().hipns_philox4x32(ptr(c), 1, key[0], key[1], ptr(o), _stream()))
+        got = o.cpu().numpy().view(np.uint32)
+        assert [int(x) for x in got] == want
+
+
+def test_ndtri_f32_accuracy():
+    """Device q = ndtri against scipy (FP64): relative 2e-6 in the body, absolute 2e-5 in the far tails."""
+    from scipy.special import ndtri
+    torch = _torch()
+    u = np.concatenate([np.linspace(1e-6, 1 - 1e-6, 100001), np.logspace(-38, -1, 3000),
+                        1 - np.logspace(-7.2, -1, 1000)]).astype(np.float32)
+    u = u[(u > 0) & (u < 1)]
+    ud = torch.from_numpy(u).cuda()
+    z = torch.empty_like(ud)
+    check(lib().hipns_ndtri_f32(ptr(ud), ud.numel(), ptr(z), _stream()))
+    got = z.cpu().numpy().astype(np.float64)
+    ref = ndtri(u.astype(np.float64))
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
+    say("ndtri max scaled err", err.max())
+    assert err.max() < 2e-6
+
+
+def _evidence_inputs(M, seed):
+    rng = np.random.default_rng(seed)
+    logL = np.sort(rng.normal(size=M).cumsum() * 0.01 + rng.normal(size=M) * 0.1)
+    logL[: min(5, M)] = -np.inf if M > 10 else logL[: min(5, M)]
+    n = rng.integers(1, 300, size=M).astype(np.float64)
+    return logL, n
+
+
+@pytest.mark.parametrize("M,stop", [(1, None), (7, None), (2048, None), (5000, None), (20000, 12345), (70001, None)])
+def test_evidence_scan_vs_oracle(M, stop):
+    torch = _torch()
+    say(f"evidence_scan M={M} stop={stop}")
+    logL, n = _evidence_inputs(M, M)
+    fin_o, per_o = O.compute_evidence_stats(logL, n, stop)
+    L = torch.from_numpy(logL).cuda()
+    N = torch.from_numpy(n).cuda()
+    per = torch.empty((8, M), dtype=torch.float64, device="cuda")
+    fin = torch.empty(8, dtype=torch.float64, device="cuda")
+    ws = torch.empty(lib().hipns_evidence_scan_workspace_size(M), dtype=torch.uint8, device="cuda")
+    check(lib().hipns_evidence_scan(ptr(L), ptr(N), M, -1 if stop is None else stop, ptr(per), ptr(fin), ptr(ws),
+                                    ws.numel(), _stream()))
+    per = per.cpu().numpy().T
+    fin = fin.cpu().numpy()
+    # same values up to FP64 reassociation of the log-space sums
+    for a, b in ((per, per_o), (fin[None], fin_o[None])):
+        fa, fb = np.isfinite(a), np.isfinite(b)
+        assert np.array_equal(fa, fb)
+        assert np.array_equal(a[~fa], b[~fb])
+        np.testing.assert_allclose(a[fa], b[fb], rtol=1e-9, atol=1e-9)
+
+
+def test_argsort_stable_vs_numpy():
+    torch = _torch()
+    rng = np.random.default_rng(3)
+    for n in [1, 5, 2048, 2049, 10000, 100003]:
+        say(f"argsort n={n}")
+        keys = np.round(rng.normal(size=n), 1)  # many ties
+        keys[rng.integers(0, n, size=max(1, n // 50))] = -np.inf
+        keys[rng.integers(0, n, size=max(1, n // 50))] = np.inf
+        k = torch.from_numpy(keys).cuda()
+        p = torch.empty(n, dtype=torch.int64, device="cuda")
+        ws = torch.empty(lib().hipns_argsort_workspace_size(n), dtype=torch.uint8, device="cuda")
+        check(lib().hipns_argsort_stable(ptr(k), n, ptr(p), ptr(ws), ws.numel(), _stream()))
+        assert np.array_equal(p.cpu().numpy(), np.argsort(keys, kind="stable"))
+
+
+def _tree(n, seed):
+    rng = np.random.default_rng(seed)
+    logL = [0.0]
+    parent = []
+    for _ in range(n):
+        i = int(rng.integers(0, len(logL)))
+        logL.append(logL[i] + float(rng.uniform(0.0, 1.0)))
+        parent.append(i)
+    return np.array(parent, np.int64), np.array(logL[1:])
+
+
+@pytest.mark.parametrize("n,num_samples", [(6, None), (200, None), (50000, None), (3000, 2500)])
+def test_count_crossed_edges_vs_oracle(n, num_samples):
+    torch = _torch()
+    say(f"count_crossed_edges n={n}")
+    sender, logL = _tree(n, n)
+    if num_samples is not None:
+        sender[num_samples:] = 0
+        logL[num_samples:] = np.inf
+    si_o, nl_o = O.count_crossed_edges(sender, logL, num_samples)
+    s = torch.from_numpy(sender.astype(np.int32)).cuda()
+    L = torch.from_numpy(logL).cuda()
+    si = torch.empty(n, dtype=torch.int64, device="cuda")
+    nl = torch.empty(n, dtype=torch.int64, device="cuda")
+    ws = torch.empty(lib().hipns_count_crossed_edges_workspace_size(n), dtype=torch.uint8, device="cuda")
+    check(lib().hipns_count_crossed_edges(ptr(s), ptr(L), n, -1 if num_samples is None else num_samples, ptr(si),
+                                          ptr(nl), ptr(ws), ws.numel(), _stream()))
+    assert np.array_equal(si.cpu().numpy(), si_o)
+    assert np.array_equal(nl.cpu().numpy(), nl_o)
+
+
+def _batch(model, N=64, num_slices=None, k=0, seed=1, lpc=2, max_ms=None):
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    D = model.U_ndims
+    spec = RunSpec(model=model, num_live_points=N, num_slices=num_slices or 2 * D, k_phantom=k,
+                   max_samples=max_ms or N * 20, seed=seed)
+    return DeviceBatch([spec], lanes_per_chain=lpc, waves_per_block=2, max_steps=64)
+
+
+@pytest.mark.parametrize("fam,D,lpc", [("mvn", 10, 2), ("mvn", 32, 2), ("mvn", 32, 4), ("mvn", 70, 2),
+                                       ("gmm", 7, 2), ("gmm", 40, 4)])
+def test_batched_loglik_vs_oracle(fam, D, lpc):
+    """Model.forward on the device (FP32) vs the FP64 oracle: 1e-5 relative."""
+    torch = _torch()
+    say(f"loglik {fam} D={D} lpc={lpc}")
+    if fam == "mvn":
+        model, _, mu, Sig = mvn_problem(D, seed=D)
+        spec = mvn_oracle_spec(D, mu, Sig)
+    else:
+        model, _, means, sig = gmm_problem(D, seed=D)
+        spec = gmm_oracle_spec(D, means, sig)
+    b = _batch(model, lpc=lpc)
+    rng = np.random.default_rng(0)
+    B = 1000
+    U = rng.uniform(0.02, 0.98, size=(B, D)).astype(np.float32)
+    if fam == "gmm":  # points near the modes so that the mixture is not -inf in FP32
+        U = ((means[rng.integers(0, means.shape[0], B)] + rng.normal(size=(B, D)) * 0.5 + 10) / 20).clip(0.01, 0.99)
+        U = U.astype(np.float32)
+    Up = np.full((B, b.dps), 0.5, np.float32)
+    Up[:, :D] = U
+    Ud = torch.from_numpy(Up).cuda()
+    out = torch.empty(B, dtype=torch.float64, device="cuda")
+    check(lib().hipns_log_likelihood(ptr(Ud), B, ptr(out), ptr(b.runs_dev), b.runs_host, C.byref(b.bufs),
+                                     C.byref(b.launch), _stream()))
+    got = out.cpu().numpy()
+    ref = spec.log_likelihood(U.astype(np.float64))
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
+    say("max rel err", rel.max())
+    assert rel.max() < 1e-5
+
+
+@pytest.mark.parametrize("D,lpc,k", [(10, 2, 0), (32, 2, 2), (32, 4, 0), (5, 4, 1)])
+def test_slice_chains_injected_bit_exact(D, lpc, k):
+    """UniDimSliceSampler chains with injected draws: the device trace (t of every pick),
+    the accept/reject sequence, eval counts and the accepted points are bit-exact with
+    the binary32 oracle replaying the device's likelihood values; the device likelihood
+    itself agrees with the FP64 oracle to 1e-5."""
+    torch = _torch()
+    say(f"slice injected D={D} lpc={lpc} k={k}")
+    model, _, mu, Sig = mvn_problem(D, seed=100 + D)
+    spec = mvn_oracle_spec(D, mu, Sig)
+    N = 64
+    ns = 2 * D
+    b = _batch(model, N=N, num_slices=ns, k=k, seed=7, lpc=lpc)
+    b.init_state()
+    torch.cuda.synchronize()
+    live_U = b.live_U[0][:N, :D].cpu().numpy()
+    live_L = b.live_logL[0][:N].cpu().numpy()
+    contour = float(live_L[N // 2 - 1])
+    rng = np.random.default_rng(D + lpc)
+    Cn = N // 2
+    seeds = np.array([O.seed_index(live_L, contour, u) for u in rng.uniform(size=Cn)], np.int32)
+    max_steps = 64
+    z = rng.normal(size=(Cn, ns, D))
+    dirs = (z / np.linalg.norm(z, axis=-1, keepdims=True)).astype(np.float32)
+    dp = np.zeros((Cn, ns, b.dps), np.float32)
+    dp[..., :D] = dirs
+    tu = rng.uniform(size=(Cn, ns, max_steps)).astype(np.float32)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    sd, dd, td = dev(seeds), dev(dp), dev(tu)
+    tr_t = torch.full((Cn, ns, max_steps), float("nan"), dtype=torch.float32, device="cuda")
+    tr_L = torch.full((Cn, ns, max_steps), float("nan"), dtype=torch.float64, device="cuda")
+    steps = torch.zeros((Cn, ns), dtype=torch.int32, device="cuda")
+    kk = max(k, 1)
+    phU = torch.zeros((Cn, kk, b.dps), dtype=torch.float32, device="cuda")
+    phL = torch.zeros((Cn, kk), dtype=torch.float64, device="cuda")
+    launch = _lib.Launch(_lib.FAM_MVN, lpc, 2, 1, 0, max_steps)
+    check(lib().hipns_slice_chains_injected(ptr(b.runs_dev), b.runs_host, C.byref(b.bufs), C.byref(launch), 0, Cn,
+                                            contour, ptr(sd), ptr(dd), ptr(td), ptr(tr_t), ptr(tr_L), ptr(steps),
+                                            ptr(phU), ptr(phL), _stream()))
+    torch.cuda.synchronize()
+    g_U = b.new_U[:Cn, :D].cpu().numpy()
+    g_L = b.new_logL[:Cn].cpu().numpy()
+    g_E = b.new_evals[:Cn].cpu().numpy()
+    tr_t, tr_L, steps = tr_t.cpu().numpy(), tr_L.cpu().numpy(), steps.cpu().numpy()
+    phU, phL = phU.cpu().numpy(), phL.cpu().numpy()
+    for c in range(Cn):
+        o = O.slice_chain(spec, live_U[seeds[c]], live_L[seeds[c]], contour, ns, dirs[c], tu[c], k_phantom=k,
+                          dtype=np.float32, logL_replay=tr_L[c])
+        assert np.array_equal(o["steps"], steps[c]), c
+        m = ~np.isnan(o["trace_t"])
+        assert np.array_equal(o["trace_t"][m], tr_t[c][m]), c
+        assert np.array_equal(o["U"], g_U[c]), c
+        assert o["logL"] == g_L[c]
+        assert o["evals"] == g_E[c]
+        if k:
+            assert np.array_equal(o["phantom_U"], phU[c, :k, :D])
+            assert np.array_equal(o["phantom_logL"], phL[c, :k])
+        # the device likelihood values vs the FP64 oracle at the same (float32) points
+        o64 = O.slice_chain(spec, live_U[seeds[c]], live_L[seeds[c]], contour, ns, dirs[c], tu[c], k_phantom=k,
+                            dtype=np.float32, logL_replay=None)
+        if np.array_equal(o64["steps"], o["steps"]):
+            mm = ~np.isnan(o64["trace_logL"])
+            rel = np.abs(o64["trace_logL"][mm] - tr_L[c][mm]) / np.maximum(np.abs(o64["trace_logL"][mm]), 1.0)
+            assert rel.max() < 1e-5
+    # accepted points satisfy the contour (uni_slice_sampler.py:160-166)
+    assert np.all(g_L > contour)
+
+
+def test_full_run_config1_logZ():
+    """Plain single run, D=10 MVN, N=300, s=5: log Z vs analytic within 4 sigma + 0.1."""
+    import jaxns_amd as J
+    say("full run config 1")
+    model, log_Z, _, _ = mvn_problem(10, seed=0)
+    ns = J.NestedSampler(model, num_live_points=300, s=5, k=0, max_samples=300 * 200)
+    reason, state = ns(1)
+    res = ns.to_results(reason, state)
+    say("logZ", res.log_Z_mean, "+-", res.log_Z_uncert, "true", log_Z, "reason", reason, "evals",
+        res.total_num_likelihood_evaluations)
+    assert np.isfinite(res.log_Z_mean)
+    assert abs(res.log_Z_mean - log_Z) < 4 * res.log_Z_uncert + 0.1
+    assert reason & 4  # dlogZ
+
+
+def test_batched_runs_match_single_runs():
+    """A run inside a batch (graph-captured shells) is bit-identical to the same seed run alone."""
+    import jaxns_amd as J
+    say("batch vs single")
+    model, log_Z, _, _ = mvn_problem(8, seed=3)
+    ns = J.NestedSampler(model, num_live_points=128, s=3, max_samples=128 * 200)
+    r1, s1 = ns(11)
+    a = ns.to_results(r1, s1)
+    rs, sts = ns.run_batch([10, 11, 12], graph_iters=4)
+    b = ns.to_results(rs[1], sts[1])
+    assert rs[1] == r1
+    assert np.array_equal(a.log_L_samples, b.log_L_samples)
+    assert np.array_equal(a.U_samples, b.U_samples)
+    assert a.log_Z_mean == b.log_Z_mean

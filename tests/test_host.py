@@ -1,0 +1,161 @@
+"""Host-side logic (no GPU): C-ABI library, model parsing and parameter packing,
+public defaults, termination conditions and the results file format."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.special import ndtri
+
+import jaxns_amd as J
+from jaxns_amd import _lib
+from jaxns_amd import distributions as tfpd
+from jaxns_amd.engine import RunSpec, flatten_term_cond, round_up_max_samples, round_up_num_live_points
+
+from problems import gmm_problem, mvn_problem
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "hipns.h")).read()
+    declared = set(re.findall(r"^\w[\w\s\*]*?\b(hipns_\w+)\(", hdr, flags=re.M))
+    assert declared, "no declarations parsed"
+    L = _lib.lib()
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert set(_lib.EXPORTED) <= declared
+    assert L.hipns_abi_version() == 1
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libhipns.so")
+    with pytest.raises(_lib.HipnsError):
+        _lib.lib()
+
+
+def test_struct_layouts():
+    assert C.sizeof(_lib.RunDesc) == 88
+    assert C.sizeof(_lib.RunState) == 264
+    assert C.sizeof(_lib.TermCond) == 10 * 4 + 10 * 8
+
+
+def test_round_up_kats():
+    # nested_samplers/sharded/tests/test_sharded_static.py:4-7
+    assert round_up_num_live_points(10, 0.5, 1) == 10
+    assert round_up_num_live_points(10, 0.5, 2) == 12
+    assert round_up_num_live_points(10, 0.5, 3) == 12
+    assert round_up_max_samples(1001, 150, 0) == 1050
+    assert round_up_max_samples(1000, 150, 1) == 1200
+
+
+def test_public_defaults():
+    # public.py:69-107 and ShardedStaticNestedSampler.__post_init__ (sharded_static.py:625-647)
+    model, _, _, _ = mvn_problem(10)
+    ns = J.NestedSampler(model, num_live_points=300, s=5)
+    assert ns.num_slices == 50 and ns.k == 0 and ns.c == 300 and ns.num_live_points == 300
+    assert ns.max_samples == 300 * 100
+    ns2 = J.NestedSampler(model)
+    assert ns2.c == 300 and ns2.num_slices == 50
+    ns3 = J.NestedSampler(model, difficult_model=True, parameter_estimation=True)
+    assert ns3.num_slices == 100 and ns3.k == 10 and ns3.c == 1000
+    with pytest.raises(ValueError):
+        J.NestedSampler(model, k=50)
+    t = ns._term(None)
+    assert t["dlogZ"] == pytest.approx(np.log(1 + 1e-3)) and t["max_samples"] == 30000
+
+
+def test_term_cond_combinators():
+    a = J.TerminationCondition(dlogZ=0.1)
+    b = J.TerminationCondition(max_samples=10)
+    assert isinstance(a & b, J.TerminationConditionConjunction)
+    assert isinstance(a | b, J.TerminationConditionDisjunction)
+    assert flatten_term_cond(a) == {"dlogZ": 0.1}
+
+
+def test_model_parsing_and_transform():
+    def prior_model():
+        x = yield J.Prior(tfpd.Uniform(low=0.0, high=2.0), name="x")
+        y = yield J.Prior(tfpd.Normal(loc=[1.0, 2.0], scale=[3.0, 4.0]), name="y")
+        return x, y
+
+    m = J.Model(prior_model, J.MultivariateNormalLikelihood(np.zeros(3), cov=np.eye(3)))
+    assert m.U_ndims == 3
+    U = np.array([[0.25, 0.5, 0.975]])
+    X = m.transform(U)
+    np.testing.assert_allclose(X["x"], [[0.5]])
+    np.testing.assert_allclose(X["y"], [[1.0, 2.0 + 4.0 * ndtri(0.975)]])
+    with pytest.raises(J.InvalidPriorName):
+        def bad():
+            yield J.Prior(tfpd.Normal(0.0, 1.0), name="a")
+            yield J.Prior(tfpd.Normal(0.0, 1.0), name="a")
+            return None
+        J.Model(bad, J.MultivariateNormalLikelihood(np.zeros(2), cov=np.eye(2)))
+
+
+def test_mvn_device_params_reproduce_likelihood():
+    """The packed (W, b, c) satisfy c - |W q(u) + b|^2/2 = log N(x(u); mu, Sigma) in FP64."""
+    def prior_model():
+        x = yield J.Prior(tfpd.MultivariateNormalTriL(loc=np.full(3, 15.0),
+                                                      scale_tril=np.array([[1, 0, 0], [.5, 1, 0], [.2, .3, 2.]])))
+        return x
+
+    Sig = np.full((3, 3), 0.99) + 0.01 * np.eye(3)
+    lik = J.MultivariateNormalLikelihood(np.zeros(3), cov=Sig)
+    m = J.Model(prior_model, lik)
+    prm, c = lik.device_params(m, 32)
+    q = prm[:32]
+    b = prm[32:64].astype(np.float64)
+    W = prm[64:].reshape(32, 32).astype(np.float64)
+    rng = np.random.default_rng(0)
+    U = rng.uniform(0.05, 0.95, size=(50, 3))
+    z = np.zeros((50, 32))
+    z[:, :3] = np.where(q[:3] == 1, ndtri(U), U)
+    r = z @ W.T + b
+    got = c - 0.5 * np.sum(r * r, axis=1)
+    want = m.forward_numpy(U)
+    np.testing.assert_allclose(got, want, rtol=1e-5)
+
+
+def test_gmm_params_and_analytic_evidence():
+    model, log_Z, means, sig = gmm_problem(2, seed=1)
+    # brute-force quadrature of the analytic mixture evidence in D=2
+    g = np.linspace(-10, 10, 1201)
+    X = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
+    L = model.log_likelihood.log_prob_numpy(X)
+    dx = g[1] - g[0]
+    bf = np.log(np.sum(np.exp(L)) * dx * dx) - 2 * np.log(20.0)
+    assert abs(bf - log_Z) < 1e-3
+    prm, _ = model.log_likelihood.device_params(model, 32)
+    assert prm.size == (3 + 4) * 32 + 8
+
+
+def test_runspec_rounding():
+    model, _, _, _ = mvn_problem(4)
+    s = RunSpec(model=model, num_live_points=301, num_slices=20, k_phantom=2, max_samples=1000)
+    assert s.shell == 150 and s.max_samples % (150 * 3) == 0
+
+
+def test_results_json_roundtrip(tmp_path):
+    res = J.NestedSamplerResults(
+        log_Z_mean=-1.5, log_Z_uncert=0.1, ESS=100.0, H_mean=-2.0, samples={"x": np.arange(6.0).reshape(3, 2)},
+        parametrised_samples={}, U_samples=np.random.default_rng(0).uniform(size=(3, 2)),
+        log_L_samples=np.array([-np.inf, 0.0, 1.0]), log_dp_mean=np.array([-np.inf, -1.0, -0.5]),
+        log_X_mean=np.array([-0.1, -0.2, -0.3]), log_posterior_density=np.zeros(3),
+        num_live_points_per_sample=np.array([3, 2, 1]), num_likelihood_evaluations_per_sample=np.array([1, 5, 7]),
+        total_num_samples=3, total_phantom_samples=0, total_num_likelihood_evaluations=13, log_efficiency=-1.0,
+        termination_reason=4)
+    f = str(tmp_path / "r.json")
+    J.save_results(res, f)
+    back = J.load_results(f)
+    for k in res._fields:
+        a, b = getattr(res, k), getattr(back, k)
+        if isinstance(a, dict):
+            for kk in a:
+                np.testing.assert_array_equal(a[kk], b[kk])
+        else:
+            np.testing.assert_array_equal(a, b)
+    out = J.summary(back)
+    assert "logZ=-1.50 +- 0.10" in out
