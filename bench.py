@@ -119,6 +119,7 @@ def cpu_baseline(mu, Sig, cfg, gpu_evals_per_run, budget_s=15.0):
         r = O.ns_run(spec, num_live_points=cfg["N"], num_slices=D * cfg["s"], max_samples=cfg["N"] * 120, seed=5,
                      num_threads=threads, max_iterations=iters)
         dt = time.perf_counter() - t0
+        print(f"[bench] cpu baseline: {iters} shells in {dt:.2f} s", file=sys.stderr, flush=True)
         if dt > budget_s / 3 or iters >= 64:
             break
         iters *= 2
@@ -129,6 +130,10 @@ def cpu_baseline(mu, Sig, cfg, gpu_evals_per_run, budget_s=15.0):
                 sample=f"{iters} shells of one run of the same workload (create_init_state + {iters} x "
                        f"_collect_shell + to_results), {evals} likelihood evals in {dt:.2f} s, "
                        f"extrapolated with {gpu_evals_per_run:.3g} evals per full run")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def main():
@@ -146,7 +151,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this build has one GPU; "
-              "multi-GPU runs would shard independent runs across processes (see DESIGN.md)", file=sys.stderr)
+              "multi-GPU runs would shard independent runs across processes", file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
@@ -157,7 +162,9 @@ def main():
     # ---------------- chosen form: R runs per step
     for w in range(args.warmup):
         b = make_batch(model, cfg, range(1000 + w * R, 1000 + (w + 1) * R), args.lpc, args.wpb)
+        t0 = time.perf_counter()
         run_batch(b, args.graph_iters)
+        log(f"warmup {w}: {R} runs in {time.perf_counter() - t0:.2f} s")
         del b
     torch.cuda.synchronize()
     logZ = []
@@ -172,6 +179,7 @@ def main():
         t_total += time.perf_counter() - t0
         logZ.append(z)
         evals_total += sum(st.num_likelihood_evaluations for st in b.states())
+        log(f"step {k}: {R} runs, total {t_total:.2f} s, mean logZ {z[:, 0].mean():.3f} (true {log_Z_true:.3f})")
         del b
     logZ = np.concatenate(logZ)
     ms_per_step = 1e3 * t_total / args.steps
@@ -190,11 +198,13 @@ def main():
         pz.append(run_batch(b, 0)[0])
         torch.cuda.synchronize()
         tp += time.perf_counter() - t0
+        log(f"plain run {k}: {tp:.2f} s so far")
         del b
     plain_runs_per_s = args.plain_runs / tp
 
     # ---------------- roofline of the dominant kernel (hipns_shell_sample)
     durs, evs = instrumented_slice_timing(model, cfg, R, args.lpc, args.wpb, seed0=90000)
+    log(f"instrumented: {durs.size} sample launches, mean {durs.mean() * 1e3:.3f} ms")
     D = cfg["D"]
     flops = evs * 2.0 * D * D  # MVN residual GEMM: 2 D^2 flops per likelihood evaluation
     avg_dur = float(durs.mean())
@@ -206,6 +216,7 @@ def main():
                     evals_per_s_in_kernel=float(evs.sum() / durs.sum()))
 
     cpu = None if args.no_cpu else cpu_baseline(mu, Sig, cfg, evals_per_run)
+    log("cpu baseline", cpu)
     out = {
         "metric": "runs/s to |dlogZ|<0.1 nat (chosen batched form, 1 GPU)",
         "value": runs_per_s,
