@@ -141,13 +141,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--runs", type=int, default=64, help="runs per step (chosen form)")
+    ap.add_argument("--runs", type=int, default=128, help="runs per step (chosen form)")
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--graph-iters", type=int, default=8)
     ap.add_argument("--lpc", type=int, default=4)
     ap.add_argument("--wpb", type=int, default=4)
     ap.add_argument("--plain-runs", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="use --lpc/--wpb instead of the autotuner")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="only the instrumented shell-by-shell pass (for rocprofv3 kernel statistics)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this build has one GPU; "
@@ -158,6 +161,20 @@ def main():
     cfg = CONFIGS[args.config]
     model, log_Z_true, mu, Sig = mvn_model(cfg["D"], seed=0)
     R = args.runs
+    if args.profile_pass:
+        durs, evs = instrumented_slice_timing(model, cfg, R, args.lpc, args.wpb, seed0=90000)
+        D = cfg["D"]
+        print(json.dumps({"profile_pass": True, "runs": R, "lanes_per_chain": args.lpc, "waves_per_block": args.wpb,
+                          "sample_launches": int(durs.size), "avg_launch_ms": float(durs.mean() * 1e3),
+                          "evals_per_launch": float(evs.mean()),
+                          "achieved_TFLOPs": float((evs * 2.0 * D * D).mean() / durs.mean() / 1e12)}), flush=True)
+        return
+    if not args.no_autotune:
+        from jaxns_amd.autotune import autotune
+        t0 = time.perf_counter()
+        args.lpc, args.wpb = autotune(model, cfg["N"], cfg["D"] * cfg["s"], R, verbose=True)
+        log(f"autotune picked lanes_per_chain={args.lpc} waves_per_block={args.wpb} "
+            f"in {time.perf_counter() - t0:.1f} s")
 
     # ---------------- chosen form: R runs per step
     for w in range(args.warmup):
@@ -174,9 +191,12 @@ def main():
         b = make_batch(model, cfg, range(k * R, (k + 1) * R), args.lpc, args.wpb)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        z = run_batch(b, args.graph_iters)
+        b.run(term_for(b), init_efficiency_threshold=0.1, graph_iters=args.graph_iters)
+        t1 = time.perf_counter()
+        z = b.log_Z()
         torch.cuda.synchronize()
         t_total += time.perf_counter() - t0
+        log(f"step {k}: sampling {t1 - t0:.3f} s, log Z of {R} runs {time.perf_counter() - t1:.3f} s")
         logZ.append(z)
         evals_total += sum(st.num_likelihood_evaluations for st in b.states())
         log(f"step {k}: {R} runs, total {t_total:.2f} s, mean logZ {z[:, 0].mean():.3f} (true {log_Z_true:.3f})")

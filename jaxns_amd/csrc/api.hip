@@ -267,7 +267,7 @@ int hipns_shell_register(const hipns_run_desc_t* runs_dev, hipns_run_state_t* st
   int rc, maxD, maxN, maxS;
   if ((rc = shell_common(runs_dev, states_dev, R, runs_host, bufs, nullptr, &maxD, &maxN, &maxS))) return rc;
   if (!term) return fail(HIPNS_EINVAL, "null termination condition");
-  hipLaunchKernelGGL(hipns::k_register, dim3(R), dim3(256), 0, as_stream(stream), runs_dev, states_dev, *bufs, *term);
+  hipLaunchKernelGGL(hipns::k_register, dim3(R), dim3(1024), 0, as_stream(stream), runs_dev, states_dev, *bufs, *term);
   HIPNS_CHECK_LAUNCH("k_register");
   return HIPNS_OK;
 }

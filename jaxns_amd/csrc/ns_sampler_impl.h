@@ -85,7 +85,7 @@ struct GmmLds {
 // r = W q(u) + b ; logL = c - |r|^2 / 2 ; W rows (outputs) tiled by 32 (LPC=2) or 16 (LPC=4)
 template <int LPC, int NJ>
 __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, int dps, const MvnLds& P,
-                                          double log_const) {
+                                          double log_const, uint32_t tfm) {
   constexpr int CW = HIPNS_WAVE / LPC;
   const int l = lane_id();
   const int m = l % CW;
@@ -97,7 +97,7 @@ __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, 
     float v = 0.0f;
     if (j < nj) {
       float u = up[j];
-      v = (P.tf[d] != 0.0f) ? ndtri_f32(u) : u;
+      v = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
       if (d >= D) v = 0.0f;
     }
     z[j] = v;
@@ -149,7 +149,7 @@ __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, 
 }
 
 template <int LPC, int NJ>
-__device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, const GmmLds& P) {
+__device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, const GmmLds& P, uint32_t tfm) {
   constexpr int CW = HIPNS_WAVE / LPC;
   const int h = lane_id() / CW;
   float part[HIPNS_MAX_GMM_K];
@@ -161,7 +161,7 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
       int d = owned_dim<LPC>(h, j);
       if (d < D) {
         float u = up[j];
-        float qv = (P.tf[d] != 0.0f) ? ndtri_f32(u) : u;
+        float qv = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
         float x = fmaf(P.q[d], qv, P.p[d]);
 #pragma unroll
         for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
@@ -183,6 +183,16 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
     }
   }
   return __builtin_isnan(acc) ? -__builtin_inf() : acc;
+}
+
+// bit j set when owned dimension j of lane-part h uses q = ndtri
+template <int LPC, int NJ>
+__device__ __forceinline__ uint32_t transform_mask(const float* tf, int h, int nj) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+    if (j < nj && tf[owned_dim<LPC>(h, j)] != 0.0f) m |= 1u << j;
+  return m;
 }
 
 // ---------------------------------------------------------------------------
@@ -251,6 +261,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   MvnLds mv;
   GmmLds gm;
   load_params(FAM, A.B.params + run.param_off, dps, D, A.gmm_K, lds, mv, gm, LPC);
+  const uint32_t tfm = transform_mask<LPC, NJ>(FAM == HIPNS_FAM_MVN ? mv.tf : gm.tf, h, nj);
 
   const int chain = chain0 + c;
   bool alive = chain < nchains;
@@ -338,9 +349,9 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
         }
       }
       ss = chain_sum<LPC>(ss);
-      float nrm = sqrtf(ss);
+      const float inv = 1.0f / sqrtf(ss);
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) nv[j] = nv[j] / nrm;
+      for (int j = 0; j < NJ; ++j) nv[j] = nv[j] * inv;
     }
     // _slice_bounds (uni_slice_sampler.py:41-64)
     float rmin = __builtin_inff(), lmax = -__builtin_inff();
@@ -393,9 +404,9 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
     // ---- batched likelihood of the wavefront's CW proposals (Model.forward)
     double L;
     if (FAM == HIPNS_FAM_MVN)
-      L = lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const);
+      L = lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const, tfm);
     else
-      L = lik_gmm<LPC, NJ>(up, nj, D, gm);
+      L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
 
     if (alive) {
       ++evals;

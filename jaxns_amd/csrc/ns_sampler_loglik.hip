@@ -32,8 +32,9 @@ __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, doubl
     int d = owned_dim<LPC>(h, j);
     up[j] = (ok && j < nj && d < D) ? U[row * dps + d] : 0.5f;
   }
-  double L = (FAM == HIPNS_FAM_MVN) ? lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const)
-                                    : lik_gmm<LPC, NJ>(up, nj, D, gm);
+  const uint32_t tfm = transform_mask<LPC, NJ>(FAM == HIPNS_FAM_MVN ? mv.tf : gm.tf, h, nj);
+  double L = (FAM == HIPNS_FAM_MVN) ? lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const, tfm)
+                                    : lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
   if (ok && h == 0) out[row] = L;
 }
 

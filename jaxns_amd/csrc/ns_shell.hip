@@ -202,10 +202,10 @@ __device__ EvMap chunk_map(const double* L, int n, double nconst, int ndesc, dou
 // ---------------------------------------------------------------------------
 // Termination register (sharded_static.py:281-323) and determine_termination.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_register(const hipns_run_desc_t* runs, hipns_run_state_t* states,
+__global__ void __launch_bounds__(1024) k_register(const hipns_run_desc_t* runs, hipns_run_state_t* states,
                                                   hipns_buffers_t B, hipns_term_cond_t term) {
   __shared__ EvMap lds_maps[16];
-  __shared__ long long red_i[16];
+  __shared__ long long red_i[32];
   __shared__ int red_p[16];
   const int r = blockIdx.x;
   hipns_run_state_t* st = states + r;
@@ -237,14 +237,14 @@ __global__ void __launch_bounds__(256) k_register(const hipns_run_desc_t* runs, 
   const int w = threadIdx.x / 64, nw = blockDim.x / 64;
   if (lane_id() == 0) {
     red_i[w] = le;
-    red_i[8 + w] = ne;
+    red_i[16 + w] = ne;
     red_p[w] = plat;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < nw; ++i) {
       le += red_i[i];
-      ne += red_i[8 + i];
+      ne += red_i[16 + i];
       plat &= red_p[i];
     }
     double lt;
