@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="use --lpc/--wpb instead of the autotuner")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only the instrumented shell-by-shell pass (for rocprofv3 kernel statistics)")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="with --profile-pass: also copy a buffer of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this build has one GPU; "
@@ -162,6 +164,14 @@ def main():
     model, log_Z_true, mu, Sig = mvn_model(cfg["D"], seed=0)
     R = args.runs
     if args.profile_pass:
+        if args.copy_calib:
+            n = args.copy_calib * (1 << 20) // 4
+            a = torch.ones(n, dtype=torch.float32, device="cuda")
+            b = torch.empty_like(a)
+            for _ in range(3):
+                b.copy_(a)
+            torch.cuda.synchronize()
+            log(f"copy calibration: 3 x {args.copy_calib} MiB read + written")
         durs, evs = instrumented_slice_timing(model, cfg, R, args.lpc, args.wpb, seed0=90000)
         D = cfg["D"]
         print(json.dumps({"profile_pass": True, "runs": R, "lanes_per_chain": args.lpc, "waves_per_block": args.wpb,

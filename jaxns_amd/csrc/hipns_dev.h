@@ -23,26 +23,52 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 // serves chain (l % CW) with part h = l / CW, CW = 64 / LPC, so partners are
 // at xor distances CW, 2 CW, ...
 // ---------------------------------------------------------------------------
+// gfx950 lane swaps: permlane16_swap exchanges the odd 16-lane rows of its
+// first operand with the even rows of its second, permlane32_swap the upper
+// half of the first with the lower half of the second.  Called with the same
+// value twice, the two results hold the two partners of every lane, so
+// r[0] (op) r[1] is the reduction over lanes l and l^16 (resp. l^32), with the
+// operands in the same order on both partners.
+__device__ __forceinline__ void swap16(float v, float& a, float& b) {
+  auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap32(float v, float& a, float& b) {
+  auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+
 template <int LPC>
 __device__ __forceinline__ float chain_sum(float v) {
-  constexpr int CW = HIPNS_WAVE / LPC;
-#pragma unroll
-  for (int o = CW; o < HIPNS_WAVE; o <<= 1) v += __shfl_xor(v, o, HIPNS_WAVE);
-  return v;
+  float a, b;
+  if constexpr (LPC == 4) {
+    swap16(v, a, b);
+    v = a + b;
+  }
+  swap32(v, a, b);
+  return a + b;
 }
 template <int LPC>
 __device__ __forceinline__ float chain_min(float v) {
-  constexpr int CW = HIPNS_WAVE / LPC;
-#pragma unroll
-  for (int o = CW; o < HIPNS_WAVE; o <<= 1) v = fminf(v, __shfl_xor(v, o, HIPNS_WAVE));
-  return v;
+  float a, b;
+  if constexpr (LPC == 4) {
+    swap16(v, a, b);
+    v = fminf(a, b);
+  }
+  swap32(v, a, b);
+  return fminf(a, b);
 }
 template <int LPC>
 __device__ __forceinline__ float chain_max(float v) {
-  constexpr int CW = HIPNS_WAVE / LPC;
-#pragma unroll
-  for (int o = CW; o < HIPNS_WAVE; o <<= 1) v = fmaxf(v, __shfl_xor(v, o, HIPNS_WAVE));
-  return v;
+  float a, b;
+  if constexpr (LPC == 4) {
+    swap16(v, a, b);
+    v = fmaxf(a, b);
+  }
+  swap32(v, a, b);
+  return fmaxf(a, b);
 }
 
 // ---------------------------------------------------------------------------
