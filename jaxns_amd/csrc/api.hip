@@ -87,10 +87,12 @@ int max_lds_bytes() {
   return v;
 }
 
-int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A, int R, int max_chains, int maxD,
-                   int dps, hipStream_t s) {
+int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A0, int R, int max_chains, int maxD,
+                   int dps, int max_slices, hipStream_t s) {
   int lpc = (dps > 64) ? 4 : L->lanes_per_chain;
-  size_t lds = hipns::sampler_lds_floats(L->family, dps, maxD, L->gmm_K, lpc) * sizeof(float);
+  hipns::SampleArgs A = A0;
+  A.alpha_off = (int32_t)hipns::sampler_lds_floats(L->family, dps, maxD, L->gmm_K, lpc);
+  size_t lds = ((size_t)A.alpha_off + (size_t)max_slices) * sizeof(float);
   if ((int)lds > max_lds_bytes())
     return fail(HIPNS_ERANGE, "model parameters need %zu B of LDS, device allows %d", lds, max_lds_bytes());
   hipns_launch_t L2 = *L;
@@ -195,7 +197,7 @@ int hipns_create_init_state(const hipns_run_desc_t* runs_dev, hipns_run_state_t*
   A.B = *bufs;
   A.max_steps = launch->max_steps;
   A.gmm_K = launch->gmm_K;
-  if ((rc = sampler_launch(hipns::MODE_INIT, launch, A, R, maxN, maxD, bufs->dps, s))) return rc;
+  if ((rc = sampler_launch(hipns::MODE_INIT, launch, A, R, maxN, maxD, bufs->dps, 0, s))) return rc;
   size_t lds = hipns::replace_lds_bytes(maxN);
   if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "N=%d needs %zu B LDS for the sort", maxN, lds);
   hipLaunchKernelGGL(hipns::k_replace, dim3(R), dim3(1024), lds, s, runs_dev, states_dev, *bufs, 1);
@@ -246,8 +248,10 @@ int hipns_shell_sample(const hipns_run_desc_t* runs_dev, hipns_run_state_t* stat
   A.B = *bufs;
   A.max_steps = launch->max_steps;
   A.gmm_K = launch->gmm_K;
+  int max_slices = 0;
+  for (int r = 0; r < R; ++r) max_slices = runs_host[r].num_slices > max_slices ? runs_host[r].num_slices : max_slices;
   return sampler_launch(sampler == 1 ? hipns::MODE_UNIF : hipns::MODE_SLICE, launch, A, R, maxS, maxD, bufs->dps,
-                        as_stream(stream));
+                        max_slices, as_stream(stream));
 }
 
 int hipns_shell_replace(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
@@ -432,7 +436,8 @@ int hipns_slice_chains_injected(const hipns_run_desc_t* runs_dev, const hipns_ru
   A.trace_steps = steps_dev;
   A.ph_U = ph_U_dev;
   A.ph_logL = ph_logL_dev;
-  return sampler_launch(hipns::MODE_INJ, launch, A, 1, C, maxD, bufs->dps, as_stream(stream));
+  return sampler_launch(hipns::MODE_INJ, launch, A, 1, C, maxD, bufs->dps, runs_host[0].num_slices,
+                        as_stream(stream));
 }
 
 int hipns_philox4x32(const uint32_t* ctr_dev, int64_t n, uint32_t key0, uint32_t key1, uint32_t* out_dev,

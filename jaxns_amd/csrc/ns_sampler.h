@@ -19,6 +19,8 @@ struct SampleArgs {
   hipns_buffers_t B;
   int32_t max_steps;
   int32_t gmm_K;
+  int32_t alpha_off;  // LDS float offset of the alpha table (after the model parameters)
+  int32_t pad0;
   // injected-draw mode
   int32_t inj_buf;
   int32_t inj_C;

@@ -311,10 +311,18 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
     for (int j = 0; j < NJ; ++j) U0[j] = 0.5f;
   }
 
-  // ---- begin a slice: direction, perfect bracket, first pick
+  // alpha = linspace(0.5, 1, num_slices)[s] (uni_slice_sampler.py:422), one table per block
+  float* alpha_tab = lds + A.alpha_off;
+  if (MODE == MODE_SLICE || MODE == MODE_INJ) {
+    for (int i = threadIdx.x; i < num_slices; i += blockDim.x)
+      alpha_tab[i] = (float)(num_slices > 1 ? 0.5 + 0.5 * (double)i / (double)(num_slices - 1) : 0.5);
+    __syncthreads();
+  }
+
+  // ---- begin a slice: direction and perfect bracket (the pick follows in the step)
   auto begin_slice = [&](int si) {
     k = 0;
-    alpha = (float)(num_slices > 1 ? 0.5 + 0.5 * (double)si / (double)(num_slices - 1) : 0.5);
+    alpha = alpha_tab[si];
     if (MODE == MODE_INJ) {
       const float* dsrc = A.inj_dirs + ((int64_t)chain * num_slices + si) * dps;
 #pragma unroll
@@ -323,29 +331,33 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
 #pragma unroll
       for (int j = 0; j < NJ; ++j) nv[j] = (owned_dim<LPC>(h, j) == 0) ? 1.0f : 0.0f;
     } else {
-      // _sample_direction (uni_slice_sampler.py:23-38): z ~ N(0, I), n = z / |z|
+      // _sample_direction (uni_slice_sampler.py:23-38): z ~ N(0, I), n = z / |z|.  One
+      // Philox block feeds the 8 owned dimensions of a group through Box-Muller on
+      // 16-bit uniform halves.
       float ss = 0.0f;
 #pragma unroll
-      for (int j = 0; j < NJ; j += 4) {
+      for (int j = 0; j < NJ; j += 8) {
         if (j < nj) {
-          int d0 = owned_dim<LPC>(h, j);
-          u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)si, RNG_DIR | (uint32_t)(d0 >> 2), k0, k1);
-          float r0 = sqrtf(-2.0f * __logf(u01f_open0(w.x)));
-          float r1 = sqrtf(-2.0f * __logf(u01f_open0(w.z)));
-          float s0, c0, s1, c1;
-          __sincosf(6.28318530717958648f * u01f(w.y), &s0, &c0);
-          __sincosf(6.28318530717958648f * u01f(w.w), &s1, &c1);
-          float zz[4] = {r0 * c0, r0 * s0, r1 * c1, r1 * s1};
+          const int d0 = owned_dim<LPC>(h, j);
+          u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)si, RNG_DIR | (uint32_t)(d0 >> 3), k0, k1);
+          const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            int d = d0 + q;
-            float v = d < D ? zz[q] : 0.0f;
-            nv[j + q] = v;
-            ss = fmaf(v, v, ss);
+            float u1 = (float)((ws[q] & 0xFFFFu) + 1u) * 1.52587890625e-5f;  // (0, 1]
+            float u2 = (float)(ws[q] >> 16) * 1.52587890625e-5f;             // [0, 1)
+            float rr = sqrtf(-2.0f * __logf(u1));
+            float sn, cs;
+            __sincosf(6.28318530717958648f * u2, &sn, &cs);
+            float z0 = d0 + 2 * q < D ? rr * cs : 0.0f;
+            float z1 = d0 + 2 * q + 1 < D ? rr * sn : 0.0f;
+            nv[j + 2 * q] = z0;
+            nv[j + 2 * q + 1] = z1;
+            ss = fmaf(z0, z0, ss);
+            ss = fmaf(z1, z1, ss);
           }
         } else {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) nv[j + q] = 0.0f;
+          for (int q = 0; q < 8; ++q) nv[j + q] = 0.0f;
         }
       }
       ss = chain_sum<LPC>(ss);
@@ -360,14 +372,6 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
       if (j < nj && owned_dim<LPC>(h, j) < D) bound_candidates(U0[j], nv[j], rmin, lmax);
     right = chain_min<LPC>(rmin);
     left = chain_max<LPC>(lmax);
-    float u;
-    if (MODE == MODE_INJ) {
-      u = A.inj_tu[((int64_t)chain * num_slices + si) * A.max_steps + 0];
-    } else {
-      u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)si, RNG_PICK, k0, k1);
-      u = u01f(w.x);
-    }
-    t = pick_t(left, right, u);
   };
 
   auto draw_uniform_point = [&](int attempt) {
@@ -382,16 +386,39 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   };
 
   bool need_begin = (MODE == MODE_SLICE || MODE == MODE_INJ) && alive;
+  bool need_shrink = false;
 
   const int64_t phantom_base = (MODE == MODE_SLICE) ? (st->next_idx + S) % run.max_samples : 0;
   const int sender = (MODE == MODE_SLICE) ? st->sender_node : 0;
   const int max_uniform = (MODE == MODE_INIT) ? 1000000 : run.max_uniform_evals;
+  u32x4 pickblk = {0u, 0u, 0u, 0u};
 
-  while (__any(alive)) {
+  // Every live chain evaluates exactly once per step, so the step index w equals each live
+  // chain's evaluation count: the pick uniform of a chain's w-th evaluation is word (w mod 4)
+  // of its Philox block w/4, drawn in the same step by the whole wavefront.
+  for (int w = 0; __any(alive); ++w) {
     if (MODE == MODE_SLICE || MODE == MODE_INJ) {
-      if (need_begin) {
-        begin_slice(s);
-        need_begin = false;
+      if (MODE == MODE_SLICE && (w & 3) == 0)
+        pickblk = philox4x32((uint32_t)chain, it32, (uint32_t)(w >> 2), RNG_PICK, k0, k1);
+      if (alive) {
+        if (need_begin) {
+          begin_slice(s);
+          need_begin = false;
+        } else if (need_shrink) {
+          // _shrink_interval (uni_slice_sampler.py:92-111)
+          shrink(t, alpha, midpoint, left, right);
+          ++k;
+        }
+        need_shrink = false;
+        float u;
+        if (MODE == MODE_INJ) {
+          u = A.inj_tu[((int64_t)chain * num_slices + s) * A.max_steps + k];
+        } else {
+          const int kw = w & 3;
+          u = u01f(kw == 0 ? pickblk.x : (kw == 1 ? pickblk.y : (kw == 2 ? pickblk.z : pickblk.w)));
+        }
+        // _pick_point_in_interval (uni_slice_sampler.py:67-89)
+        t = pick_t(left, right, u);
       }
     }
     // ---- proposal
@@ -491,17 +518,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
             need_begin = true;
           }
         } else {
-          shrink(t, alpha, midpoint, left, right);
-          ++k;
-          float u;
-          if (MODE == MODE_INJ) {
-            u = A.inj_tu[((int64_t)chain * num_slices + s) * A.max_steps + k];
-          } else {
-            u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)s, RNG_PICK | (uint32_t)(k >> 2), k0, k1);
-            const int kw = k & 3;
-            u = u01f(kw == 0 ? w.x : (kw == 1 ? w.y : (kw == 2 ? w.z : w.w)));
-          }
-          t = pick_t(left, right, u);
+          need_shrink = true;
         }
       }
     }
