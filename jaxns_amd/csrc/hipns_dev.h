@@ -214,7 +214,7 @@ __device__ __forceinline__ float ndtri_f32(float u) {
     p = fmaf(p, w, 0.246640727f);
     p = fmaf(p, w, 1.50140941f);
   } else {
-    w = sqrtf(w) - 3.0f;
+    w = __builtin_amdgcn_sqrtf(w) - 3.0f;
     p = -0.000200214257f;
     p = fmaf(p, w, 0.000100950558f);
     p = fmaf(p, w, 0.00134934322f);

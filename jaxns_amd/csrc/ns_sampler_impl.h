@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           for (int q = 0; q < 4; ++q) {
             float u1 = (float)((ws[q] & 0xFFFFu) + 1u) * 1.52587890625e-5f;  // (0, 1]
             float u2 = (float)(ws[q] >> 16) * 1.52587890625e-5f;             // [0, 1)
-            float rr = sqrtf(-2.0f * __logf(u1));
+            float rr = __builtin_amdgcn_sqrtf(-2.0f * __logf(u1));
             float sn, cs;
             __sincosf(6.28318530717958648f * u2, &sn, &cs);
             float z0 = d0 + 2 * q < D ? rr * cs : 0.0f;
@@ -360,8 +360,9 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           for (int q = 0; q < 8; ++q) nv[j + q] = 0.0f;
         }
       }
+      // only the direction matters (t scales with 1/|n|), so the hardware rsq suffices
       ss = chain_sum<LPC>(ss);
-      const float inv = 1.0f / sqrtf(ss);
+      const float inv = __builtin_amdgcn_rsqf(ss);
 #pragma unroll
       for (int j = 0; j < NJ; ++j) nv[j] = nv[j] * inv;
     }
