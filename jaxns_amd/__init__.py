@@ -11,7 +11,8 @@ kernels from libhipns.so (jaxns_amd/csrc); Python drives them through ctypes.
 """
 from . import distributions
 from .framework import InvalidPriorName, Model, Prior
-from .likelihoods import GaussianMixtureLikelihood, LinearRegressionLikelihood, MultivariateNormalLikelihood
+from .likelihoods import (GaussianMixtureLikelihood, LinearRegressionLikelihood, MultivariateNormalLikelihood,
+                          RegressionData)
 from .nested_sampler import DefaultNestedSampler, NestedSampler, NestedSamplerState
 from .types import (NestedSamplerResults, TerminationCondition, TerminationConditionConjunction,
                     TerminationConditionDisjunction)
@@ -19,5 +20,5 @@ from .utils import load_results, save_results, summary
 
 __all__ = ["Model", "Prior", "NestedSampler", "DefaultNestedSampler", "NestedSamplerState", "TerminationCondition",
            "TerminationConditionConjunction", "TerminationConditionDisjunction", "NestedSamplerResults",
-           "MultivariateNormalLikelihood", "GaussianMixtureLikelihood", "LinearRegressionLikelihood", "summary",
+           "MultivariateNormalLikelihood", "GaussianMixtureLikelihood", "LinearRegressionLikelihood", "RegressionData", "summary",
            "save_results", "load_results", "distributions", "InvalidPriorName"]

@@ -65,10 +65,12 @@ class Buffers(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("family", _i32), ("lanes_per_chain", _i32), ("waves_per_block", _i32), ("gmm_K", _i32),
-                ("lr_ndata", _i32), ("max_steps", _i32)]
+                ("lr_ndata", _i32), ("max_steps", _i32), ("lr_xstride", _i32), ("lr_terms", _i32),
+                ("lr_data_off", _i64)]
 
 
 assert C.sizeof(RunDesc) == 88, C.sizeof(RunDesc)
+assert C.sizeof(Launch) == 40, C.sizeof(Launch)
 
 _lib = None
 

@@ -1,42 +1,51 @@
 """Launch-shape autotuner for the fused sampling kernel.
 
 Candidates are (lanes_per_chain, waves_per_block): 2 lanes per chain = 32
-chains per wavefront with 32x32x2 FP32 MFMA tiles, 4 lanes per chain = 16
-chains per wavefront with 16x16x4 tiles; 1-4 wavefronts per workgroup.  Each
-candidate runs a few shells of a trial batch of the real shape (same model,
-num_live_points, slices and number of runs) and is scored by likelihood
-evaluations per second of the sampling kernel, timed with HIP events on the
-launch stream.  Results are cached per (family, D, N, slices, runs, device).
+chains per wavefront with 32x32 MFMA tiles, 4 lanes per chain = 16 chains per
+wavefront with 16x16 tiles.  For the dense-Gaussian and mixture families the
+workgroup holds 1-4 independent wavefronts of chains; for the regression family
+its 1-4 wavefronts share one group of chains and split the data rows.  Each
+candidate runs a few shells of a trial batch of the real shape (the same run
+specs, with a small sample budget) and is scored by likelihood evaluations per
+second of the sampling kernel, timed with HIP events on the launch stream.
+Results are cached per batch shape and device.
 """
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
+import hashlib
 import json
 import os
-from typing import Iterable, Optional, Tuple
-
-import numpy as np
+from typing import Iterable, List, Optional, Tuple
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), ".autotune_cache.json")
 DEFAULT_CANDIDATES = ((2, 1), (2, 2), (2, 4), (4, 1), (4, 2), (4, 4))
 
 
-def _key(model, N, num_slices, R):
+def _key(specs, lr_terms) -> str:
     import torch
     name = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "none"
-    return f"fam{model.log_likelihood.family}-D{model.U_ndims}-N{N}-S{num_slices}-R{R}-{name}"
+    lik = specs[0].model.log_likelihood
+    fam = lik.family
+    shape = [(s.model.U_ndims, s.num_live_points, s.num_slices) for s in specs]
+    extra = ""
+    if fam == 2:
+        extra = f"-rows{lik.N}-t{lr_terms}"
+    h = hashlib.sha1(repr(shape).encode()).hexdigest()[:12]
+    return f"fam{fam}{extra}-R{len(specs)}-{h}-{name}"
 
 
-def measure(model, N, num_slices, R, lpc, wpb, shells=3, seed0=777) -> float:
+def measure(specs, lpc, wpb, shells=3, lr_terms=3) -> float:
     """Likelihood evaluations per second of the sampling kernel over `shells` shells."""
     import torch
     from ._lib import check, lib, make_term_cond, ptr, stream_ptr
-    from .engine import DeviceBatch, RunSpec
-    specs = [RunSpec(model=model, num_live_points=N, num_slices=num_slices, max_samples=N * (shells + 4),
-                     seed=seed0 + r) for r in range(R)]
-    b = DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=wpb)
+    from .engine import DeviceBatch
+    trial = [dataclasses.replace(s, max_samples=s.num_live_points * (shells + 4), seed=777 + i)
+             for i, s in enumerate(specs)]
+    b = DeviceBatch(trial, lanes_per_chain=lpc, waves_per_block=wpb, lr_terms=lr_terms)
     b.init_state()
-    tc = make_term_cond(max_samples=float(N * (shells + 4)))
+    tc = make_term_cond(max_samples=float(max(s.max_samples for s in trial)))
     s = stream_ptr()
     t = 0.0
     evals = 0
@@ -59,9 +68,10 @@ def measure(model, N, num_slices, R, lpc, wpb, shells=3, seed0=777) -> float:
     return evals / t
 
 
-def autotune(model, N: int, num_slices: int, R: int, candidates: Optional[Iterable[Tuple[int, int]]] = None,
-             shells: int = 3, use_cache: bool = True, verbose: bool = False) -> Tuple[int, int]:
-    key = _key(model, N, num_slices, R)
+def autotune(specs: List, candidates: Optional[Iterable[Tuple[int, int]]] = None, shells: int = 3,
+             use_cache: bool = True, verbose: bool = False, lr_terms: int = 3) -> Tuple[int, int]:
+    """Best (lanes_per_chain, waves_per_block) for a batch of these run specs."""
+    key = _key(specs, lr_terms)
     cache = {}
     if use_cache and os.path.exists(CACHE):
         try:
@@ -73,7 +83,7 @@ def autotune(model, N: int, num_slices: int, R: int, candidates: Optional[Iterab
         return tuple(cache[key]["best"])
     scores = {}
     for lpc, wpb in (candidates or DEFAULT_CANDIDATES):
-        scores[(lpc, wpb)] = measure(model, N, num_slices, R, lpc, wpb, shells=shells)
+        scores[(lpc, wpb)] = measure(specs, lpc, wpb, shells=shells, lr_terms=lr_terms)
         if verbose:
             print(f"[autotune] lpc={lpc} wpb={wpb}: {scores[(lpc, wpb)] / 1e9:.3f} G evals/s", flush=True)
     best = max(scores, key=scores.get)

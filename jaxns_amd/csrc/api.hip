@@ -69,8 +69,14 @@ int check_runs(const hipns_run_desc_t* runs_host, int32_t R, const hipns_buffers
 
 int check_launch(const hipns_launch_t* L) {
   if (!L) return fail(HIPNS_EINVAL, "null launch config");
-  if (L->family != HIPNS_FAM_MVN && L->family != HIPNS_FAM_GMM)
+  if (L->family != HIPNS_FAM_MVN && L->family != HIPNS_FAM_GMM && L->family != HIPNS_FAM_LINREG)
     return fail(HIPNS_ENOTSUP, "family %d not supported by the fused sampler", L->family);
+  if (L->family == HIPNS_FAM_LINREG) {
+    if (L->lr_ndata < 32 || L->lr_ndata % 32) return fail(HIPNS_EINVAL, "lr_ndata must be a positive multiple of 32");
+    if (L->lr_xstride < 32 || L->lr_xstride % 32) return fail(HIPNS_EINVAL, "lr_xstride must be a multiple of 32");
+    if (L->lr_data_off < 0 || L->lr_data_off % 4) return fail(HIPNS_EINVAL, "lr_data_off must be a multiple of 4");
+    if (L->lr_terms != 0 && L->lr_terms != 3 && L->lr_terms != 4) return fail(HIPNS_EINVAL, "lr_terms must be 3 or 4");
+  }
   if (L->lanes_per_chain != 2 && L->lanes_per_chain != 4)
     return fail(HIPNS_EINVAL, "lanes_per_chain must be 2 or 4");
   if (L->waves_per_block < 1 || L->waves_per_block > 4) return fail(HIPNS_EINVAL, "waves_per_block must be 1..4");
@@ -91,7 +97,17 @@ int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A
                    int dps, int max_slices, hipStream_t s) {
   int lpc = (dps > 64) ? 4 : L->lanes_per_chain;
   hipns::SampleArgs A = A0;
-  A.alpha_off = (int32_t)hipns::sampler_lds_floats(L->family, dps, maxD, L->gmm_K, lpc);
+  if (L->family == HIPNS_FAM_LINREG) {
+    if (L->lr_xstride < dps) return fail(HIPNS_EINVAL, "lr_xstride %d below dps %d", L->lr_xstride, dps);
+    const float* base = A.B.params + L->lr_data_off;
+    A.lr_r0 = base;
+    A.lr_xh = (const uint16_t*)(base + L->lr_ndata);
+    A.lr_xl = A.lr_xh + (int64_t)L->lr_ndata * L->lr_xstride;
+    A.lr_npad = L->lr_ndata;
+    A.lr_xs = L->lr_xstride;
+    A.lr_terms = L->lr_terms == 4 ? 4 : 3;
+  }
+  A.alpha_off = (int32_t)hipns::sampler_lds_floats(L->family, dps, maxD, L->gmm_K, lpc, L->waves_per_block);
   size_t lds = ((size_t)A.alpha_off + (size_t)max_slices) * sizeof(float);
   if ((int)lds > max_lds_bytes())
     return fail(HIPNS_ERANGE, "model parameters need %zu B of LDS, device allows %d", lds, max_lds_bytes());
@@ -382,11 +398,23 @@ int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const 
   const int D = runs_host[0].D;
   if (D < 1 || D > bufs->dps) return fail(HIPNS_EINVAL, "D outside [1, dps]");
   int lpc = bufs->dps > 64 ? 4 : launch->lanes_per_chain;
-  size_t lds = hipns::sampler_lds_floats(launch->family, bufs->dps, D, launch->gmm_K, lpc) * sizeof(float);
+  size_t lds = hipns::sampler_lds_floats(launch->family, bufs->dps, D, launch->gmm_K, lpc, launch->waves_per_block) *
+               sizeof(float);
   if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "parameters exceed LDS");
   hipns_launch_t L2 = *launch;
   L2.lanes_per_chain = lpc;
-  hipError_t e = hipns::launch_loglik(U_dev, B, logL_dev, runs_dev, *bufs, L2, bufs->dps, lds, as_stream(stream));
+  hipns::LinRegLds lr{};
+  if (launch->family == HIPNS_FAM_LINREG) {
+    if (launch->lr_xstride < bufs->dps) return fail(HIPNS_EINVAL, "lr_xstride below dps");
+    const float* base = bufs->params + launch->lr_data_off;
+    lr.r0 = base;
+    lr.xh = (const uint16_t*)(base + launch->lr_ndata);
+    lr.xl = lr.xh + (int64_t)launch->lr_ndata * launch->lr_xstride;
+    lr.npad = launch->lr_ndata;
+    lr.xs = launch->lr_xstride;
+    lr.terms = launch->lr_terms == 4 ? 4 : 3;
+  }
+  hipError_t e = hipns::launch_loglik(U_dev, B, logL_dev, runs_dev, *bufs, L2, bufs->dps, lds, as_stream(stream), lr);
   if (e != hipSuccess) return hip_fail(e, "loglik launch");
   return HIPNS_OK;
 }

@@ -21,6 +21,14 @@ struct SampleArgs {
   int32_t gmm_K;
   int32_t alpha_off;  // LDS float offset of the alpha table (after the model parameters)
   int32_t pad0;
+  // regression family: shared data of the batch
+  const float* lr_r0;
+  const uint16_t* lr_xh;
+  const uint16_t* lr_xl;
+  int32_t lr_npad;
+  int32_t lr_xs;
+  int32_t lr_terms;
+  int32_t pad1;
   // injected-draw mode
   int32_t inj_buf;
   int32_t inj_C;
@@ -36,7 +44,8 @@ struct SampleArgs {
 };
 
 // floats of dynamic LDS one sampler block needs (layout of load_params in ns_sampler_impl.h)
-inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc) {
+inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc, int G = 1) {
+  if (fam == HIPNS_FAM_LINREG) return (size_t)4 * dps + 4 + (size_t)2 * G * (64 / lpc);
   if (fam == HIPNS_FAM_MVN) {
     int CW = 64 / lpc;
     int MB = (D + CW - 1) / CW;
@@ -46,8 +55,24 @@ inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc) {
   return (size_t)(3 + K) * dps + 2 * K;
 }
 
+// regression-family view of the model: per-run vectors in LDS, the shared data rows in HBM
+struct LinRegLds {
+  float* tf;           // [dps] 1 where the prior of the weight is Normal (q = ndtri(u))
+  float* p;            // [dps] w = p + s (.) q(u)
+  float* s;            // [dps]
+  float* w0;           // [dps] centre of the residual GEMM
+  float inv2s2;        // 1 / (2 sigma^2)
+  double* partial;     // [G][CW] per-wavefront partial sums of squares
+  const float* r0;     // [npad] r0 = y - X w0 (computed in FP64, zero on padding rows)
+  const uint16_t* xh;  // [npad][xs] bf16 hi part of X
+  const uint16_t* xl;  // [npad][xs] bf16 lo part of X - hi
+  int npad, xs;
+  int terms;           // 3: BF16x3; 4: adds X hi times the third bf16 part of w - w0
+};
+
 hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
-                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s);
+                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s,
+                         const LinRegLds& lr);
 
 }  // namespace hipns
 

@@ -82,6 +82,129 @@ struct GmmLds {
   int dps;
 };
 
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ double chain_sum_d_swap16(double v) {
+  float lo = __uint_as_float((uint32_t)__double_as_longlong(v)), hi = __uint_as_float((uint32_t)(__double_as_longlong(v) >> 32));
+  float a0, b0, a1, b1;
+  swap16(lo, a0, b0);
+  swap16(hi, a1, b1);
+  double A = __longlong_as_double(((long long)__float_as_uint(a1) << 32) | __float_as_uint(a0));
+  double B = __longlong_as_double(((long long)__float_as_uint(b1) << 32) | __float_as_uint(b0));
+  return A + B;
+}
+__device__ __forceinline__ double chain_sum_d_swap32(double v) {
+  float lo = __uint_as_float((uint32_t)__double_as_longlong(v)), hi = __uint_as_float((uint32_t)(__double_as_longlong(v) >> 32));
+  float a0, b0, a1, b1;
+  swap32(lo, a0, b0);
+  swap32(hi, a1, b1);
+  double A = __longlong_as_double(((long long)__float_as_uint(a1) << 32) | __float_as_uint(a0));
+  double B = __longlong_as_double(((long long)__float_as_uint(b1) << 32) | __float_as_uint(b0));
+  return A + B;
+}
+template <int LPC>
+__device__ __forceinline__ double chain_sum_d(double v) {
+  if constexpr (LPC == 4) v = chain_sum_d_swap16(v);
+  return chain_sum_d_swap32(v);
+}
+
+// Gaussian linear regression: r = r0 - X (w - w0) with w = p + s (.) q(u); logL = c - |r|^2 / (2 sigma^2).
+// The GEMM (rows x dims) x (dims x chains) runs as BF16x3 MFMAs (hi.hi + hi.lo + lo.hi, FP32
+// accumulate; terms == 4 adds hi times the third bf16 part of w - w0, which removes the error of
+// the 16-bit weight split that every row shares): LPC=2 -> v_mfma_f32_32x32x16_bf16 over 32-row tiles, LPC=4 -> v_mfma_f32_16x16x32_bf16
+// over 16-row tiles.  The G wavefronts of the block share the chains and split the rows; each row
+// tile's 16 (4) squares are summed in FP32 and accumulated in FP64.
+template <int LPC, int NJ>
+__device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int D, const LinRegLds& P, uint32_t tfm,
+                                             int g, int G, double log_const) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  constexpr int NG = NJ / 8;  // k-groups of 8 owned dims
+  const int l = lane_id();
+  const int m = l % CW;
+  const int h = l / CW;
+  bf16x8_t bh[NG], bl[NG], bt[NG];
+#pragma unroll
+  for (int gg = 0; gg < NG; ++gg) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int j = gg * 8 + i;
+      const int d = owned_dim<LPC>(h, j);
+      float dw = 0.0f;
+      if (j < nj && d < D) {
+        float u = up[j];
+        float q = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
+        dw = fmaf(P.s[d], q, P.p[d]) - P.w0[d];
+      }
+      const __bf16 hi = (__bf16)dw;
+      const float rem = dw - (float)hi;
+      const __bf16 lo = (__bf16)rem;
+      bh[gg][i] = hi;
+      bl[gg][i] = lo;
+      bt[gg][i] = (__bf16)(rem - (float)lo);
+    }
+  }
+  const bool t4 = P.terms == 4;
+  const int ngroups = (nj + 7) / 8;
+  const int tiles = P.npad / CW;
+  const int t0 = (int)(((long long)tiles * g) / G), t1 = (int)(((long long)tiles * (g + 1)) / G);
+  double ss = 0.0;
+  for (int tl = t0; tl < t1; ++tl) {
+    const int row = tl * CW + m;
+    const uint16_t* xh = P.xh + (int64_t)row * P.xs + 8 * h;
+    const uint16_t* xl = P.xl + (int64_t)row * P.xs + 8 * h;
+    float part = 0.0f;
+    if constexpr (LPC == 2) {
+      f32x16 acc = {0};
+#pragma unroll
+      for (int gg = 0; gg < NG; ++gg) {
+        if (gg < ngroups) {
+          bf16x8_t ah = __builtin_bit_cast(bf16x8_t, *(const uint4*)(xh + 16 * gg));
+          bf16x8_t al = __builtin_bit_cast(bf16x8_t, *(const uint4*)(xl + 16 * gg));
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[gg], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[gg], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[gg], acc, 0, 0, 0);
+          if (t4) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bt[gg], acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        float r = P.r0[tl * 32 + 8 * (i >> 2) + 4 * h + (i & 3)] - acc[i];
+        part = fmaf(r, r, part);
+      }
+    } else {
+      f32x4 acc = {0};
+#pragma unroll
+      for (int gg = 0; gg < NG; ++gg) {
+        if (gg < ngroups) {
+          bf16x8_t ah = __builtin_bit_cast(bf16x8_t, *(const uint4*)(xh + 32 * gg));
+          bf16x8_t al = __builtin_bit_cast(bf16x8_t, *(const uint4*)(xl + 32 * gg));
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[gg], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[gg], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[gg], acc, 0, 0, 0);
+          if (t4) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bt[gg], acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float r = P.r0[tl * 16 + 4 * h + i] - acc[i];
+        part = fmaf(r, r, part);
+      }
+    }
+    ss += (double)part;
+  }
+  ss = chain_sum_d<LPC>(ss);
+  if (G > 1) {
+    if (h == 0) P.partial[g * CW + m] = ss;
+    __syncthreads();
+    ss = 0.0;
+    for (int gg = 0; gg < G; ++gg) ss += P.partial[gg * CW + m];
+    __syncthreads();
+  }
+  double L = log_const - ss * (double)P.inv2s2;
+  return __builtin_isnan(L) ? -__builtin_inf() : L;
+}
+
 // r = W q(u) + b ; logL = c - |r|^2 / 2 ; W rows (outputs) tiled by 32 (LPC=2) or 16 (LPC=4)
 template <int LPC, int NJ>
 __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, int dps, const MvnLds& P,
@@ -202,9 +325,21 @@ __device__ __forceinline__ uint32_t transform_mask(const float* tf, int h, int n
 
 // dynamic LDS layout of one block (one run per block)
 __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, int D, int K, float* lds, MvnLds& mv,
-                                            GmmLds& gm, int LPCv) {
+                                            GmmLds& gm, int LPCv, LinRegLds* lr = nullptr) {
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
+  if (fam == HIPNS_FAM_LINREG) {
+    lr->tf = lds;
+    lr->p = lds + dps;
+    lr->s = lds + 2 * dps;
+    lr->w0 = lds + 3 * dps;
+    lr->partial = (double*)(lds + 4 * dps + 4);
+    for (int i = tid; i < 4 * dps + 4; i += nt) lds[i] = prm[i];
+    __syncthreads();
+    lr->inv2s2 = lds[4 * dps];
+    gm.tf = lr->tf;
+    return;
+  }
   if (fam == HIPNS_FAM_MVN) {
     const int CW = HIPNS_WAVE / LPCv;
     const int MB = (D + CW - 1) / CW;
@@ -240,6 +375,8 @@ __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, 
 template <int LPC, int NJ, int FAM, int MODE>
 __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   constexpr int CW = HIPNS_WAVE / LPC;
+  // the regression family splits the data rows of one chain group over the block's wavefronts
+  constexpr bool SPLITK = (FAM == HIPNS_FAM_LINREG);
   extern __shared__ float lds[];
   const int r = blockIdx.y;
   const hipns_run_desc_t run = A.runs[r];
@@ -247,12 +384,15 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   if (MODE != MODE_INJ && st->done) return;  // uniform across the block
   const int wpb = blockDim.x / HIPNS_WAVE;
   const int wave = threadIdx.x / HIPNS_WAVE;
+  const int G = SPLITK ? wpb : 1;
+  const int gw = SPLITK ? wave : 0;
+  const bool writer = gw == 0;
   const int l = lane_id();
   const int c = l % CW;
   const int h = l / CW;
   const int nchains = (MODE == MODE_INIT) ? run.N : ((MODE == MODE_INJ) ? A.inj_C : run.shell);
-  const int chain0 = (blockIdx.x * wpb + wave) * CW;
-  if (blockIdx.x * wpb * CW >= nchains) return;  // whole block idle
+  const int chain0 = SPLITK ? blockIdx.x * CW : (blockIdx.x * wpb + wave) * CW;
+  if ((SPLITK ? blockIdx.x * CW : blockIdx.x * wpb * CW) >= nchains) return;  // whole block idle
   const int dps = A.B.dps;
   const int D = run.D;
   const int DPr = ((D + 8 * LPC - 1) / (8 * LPC)) * (8 * LPC);
@@ -260,7 +400,16 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
 
   MvnLds mv;
   GmmLds gm;
-  load_params(FAM, A.B.params + run.param_off, dps, D, A.gmm_K, lds, mv, gm, LPC);
+  LinRegLds lr;
+  load_params(FAM, A.B.params + run.param_off, dps, D, A.gmm_K, lds, mv, gm, LPC, &lr);
+  if (SPLITK) {
+    lr.r0 = A.lr_r0;
+    lr.xh = A.lr_xh;
+    lr.xl = A.lr_xl;
+    lr.npad = A.lr_npad;
+    lr.xs = A.lr_xs;
+    lr.terms = A.lr_terms;
+  }
   const uint32_t tfm = transform_mask<LPC, NJ>(FAM == HIPNS_FAM_MVN ? mv.tf : gm.tf, h, nj);
 
   const int chain = chain0 + c;
@@ -397,7 +546,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   // Every live chain evaluates exactly once per step, so the step index w equals each live
   // chain's evaluation count: the pick uniform of a chain's w-th evaluation is word (w mod 4)
   // of its Philox block w/4, drawn in the same step by the whole wavefront.
-  for (int w = 0; __any(alive); ++w) {
+  for (int w = 0; SPLITK ? __syncthreads_or(alive) : __any(alive); ++w) {
     if (MODE == MODE_SLICE || MODE == MODE_INJ) {
       if (MODE == MODE_SLICE && (w & 3) == 0)
         pickblk = philox4x32((uint32_t)chain, it32, (uint32_t)(w >> 2), RNG_PICK, k0, k1);
@@ -433,14 +582,16 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
     double L;
     if (FAM == HIPNS_FAM_MVN)
       L = lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const, tfm);
-    else
+    else if (FAM == HIPNS_FAM_GMM)
       L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
+    else
+      L = lik_linreg<LPC, NJ>(up, nj, D, lr, tfm, gw, G, run.log_const);
 
     if (alive) {
       ++evals;
       if (MODE == MODE_INJ) {
         int64_t o = ((int64_t)chain * num_slices + s) * A.max_steps + k;
-        if (h == 0) {
+        if (h == 0 && writer) {
           A.trace_t[o] = t;
           A.trace_logL[o] = L;
         }
@@ -456,12 +607,14 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           alive = false;
           const int64_t o = run.chain_off + chain;
           float* dst = A.B.new_U + o * dps;
+          if (writer) {
 #pragma unroll
-          for (int j = 0; j < NJ; ++j)
-            if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
-          if (h == 0) {
-            A.B.new_logL[o] = logL0;
-            A.B.new_evals[o] = (int32_t)evals;
+            for (int j = 0; j < NJ; ++j)
+              if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+            if (h == 0) {
+              A.B.new_logL[o] = logL0;
+              A.B.new_evals[o] = (int32_t)evals;
+            }
           }
         }
       } else {
@@ -476,13 +629,13 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           ++cap_hits;
         }
         if (acc) {
-          if (MODE == MODE_INJ && h == 0) A.trace_steps[(int64_t)chain * num_slices + s] = k + 1;
+          if (MODE == MODE_INJ && h == 0 && writer) A.trace_steps[(int64_t)chain * num_slices + s] = k + 1;
 #pragma unroll
           for (int j = 0; j < NJ; ++j) U0[j] = up[j];
           logL0 = L;
           // phantom samples: x[-(k+1):-1] (uni_slice_sampler.py:432-440), evals 0
           int pidx = s - (num_slices - 1 - kph);
-          if (kph > 0 && pidx >= 0 && s < num_slices - 1) {
+          if (kph > 0 && pidx >= 0 && s < num_slices - 1 && writer) {
             if (MODE == MODE_INJ) {
               float* pd = A.ph_U + ((int64_t)chain * kph + pidx) * dps;
 #pragma unroll
@@ -508,12 +661,14 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
             alive = false;
             const int64_t o = run.chain_off + chain;
             float* dst = A.B.new_U + o * dps;
+            if (writer) {
 #pragma unroll
-            for (int j = 0; j < NJ; ++j)
-              if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
-            if (h == 0) {
-              A.B.new_logL[o] = logL0;
-              A.B.new_evals[o] = (int32_t)evals;
+              for (int j = 0; j < NJ; ++j)
+                if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+              if (h == 0) {
+                A.B.new_logL[o] = logL0;
+                A.B.new_evals[o] = (int32_t)evals;
+              }
             }
           } else {
             need_begin = true;
@@ -524,7 +679,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
       }
     }
   }
-  if (cap_hits > 0 && h == 0 && MODE != MODE_INJ) {
+  if (cap_hits > 0 && h == 0 && writer && MODE != MODE_INJ) {
     atomicAdd((unsigned long long*)&st->cap_hits, (unsigned long long)cap_hits);
     atomicOr(&st->flags, 1);
   }

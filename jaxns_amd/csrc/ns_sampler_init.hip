@@ -10,7 +10,7 @@ template <int LPC, int NJ, int FAM, int MODE>
 static hipError_t launch_one(const SampleArgs& A, int R, int max_chains, int wpb, size_t lds_bytes, hipStream_t s) {
   constexpr int CW = HIPNS_WAVE / LPC;
   int groups = (max_chains + CW - 1) / CW;
-  int bx = (groups + wpb - 1) / wpb;
+  int bx = (FAM == HIPNS_FAM_LINREG) ? groups : (groups + wpb - 1) / wpb;
   if (bx < 1) bx = 1;
   dim3 grid(bx, R), block(wpb * HIPNS_WAVE);
   hipLaunchKernelGGL((k_sample<LPC, NJ, FAM, MODE>), grid, block, lds_bytes, s, A);
@@ -37,6 +37,8 @@ static hipError_t launch_fam(const SampleArgs& A, int R, int max_chains, const h
                              size_t lds, hipStream_t s) {
   if (L.family == HIPNS_FAM_GMM)
     return launch_lpc<HIPNS_FAM_GMM, MODE>(A, R, max_chains, L.lanes_per_chain, L.waves_per_block, dpmax, lds, s);
+  if (L.family == HIPNS_FAM_LINREG)
+    return launch_lpc<HIPNS_FAM_LINREG, MODE>(A, R, max_chains, L.lanes_per_chain, L.waves_per_block, dpmax, lds, s);
   return launch_lpc<HIPNS_FAM_MVN, MODE>(A, R, max_chains, L.lanes_per_chain, L.waves_per_block, dpmax, lds, s);
 }
 

@@ -9,8 +9,9 @@ namespace hipns {
 // ---------------------------------------------------------------------------
 template <int LPC, int NJ, int FAM>
 __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
-                                                hipns_buffers_t Bf, int K) {
+                                                hipns_buffers_t Bf, int K, LinRegLds lrd) {
   constexpr int CW = HIPNS_WAVE / LPC;
+  constexpr bool SPLITK = (FAM == HIPNS_FAM_LINREG);
   extern __shared__ float lds[];
   const hipns_run_desc_t run = runs[0];
   const int dps = Bf.dps;
@@ -19,12 +20,13 @@ __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, doubl
   const int nj = DPr / LPC;
   MvnLds mv;
   GmmLds gm;
-  load_params(FAM, Bf.params + run.param_off, dps, D, K, lds, mv, gm, LPC);
+  LinRegLds lr = lrd;
+  load_params(FAM, Bf.params + run.param_off, dps, D, K, lds, mv, gm, LPC, &lr);
   const int wpb = blockDim.x / HIPNS_WAVE;
   const int wave = threadIdx.x / HIPNS_WAVE;
   const int l = lane_id();
   const int c = l % CW, h = l / CW;
-  const int64_t row = ((int64_t)blockIdx.x * wpb + wave) * CW + c;
+  const int64_t row = SPLITK ? (int64_t)blockIdx.x * CW + c : ((int64_t)blockIdx.x * wpb + wave) * CW + c;
   float up[NJ];
   const bool ok = row < B;
 #pragma unroll
@@ -33,37 +35,47 @@ __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, doubl
     up[j] = (ok && j < nj && d < D) ? U[row * dps + d] : 0.5f;
   }
   const uint32_t tfm = transform_mask<LPC, NJ>(FAM == HIPNS_FAM_MVN ? mv.tf : gm.tf, h, nj);
-  double L = (FAM == HIPNS_FAM_MVN) ? lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const, tfm)
-                                    : lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
-  if (ok && h == 0) out[row] = L;
+  double L;
+  if (FAM == HIPNS_FAM_MVN)
+    L = lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const, tfm);
+  else if (FAM == HIPNS_FAM_GMM)
+    L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
+  else
+    L = lik_linreg<LPC, NJ>(up, nj, D, lr, tfm, wave, wpb, run.log_const);
+  if (ok && h == 0 && (!SPLITK || wave == 0)) out[row] = L;
 }
 
 template <int LPC, int FAM>
 static hipError_t launch_loglik_dp(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
-                                   const hipns_buffers_t& Bf, int K, int wpb, int dpmax, size_t lds, hipStream_t s) {
-  if (LPC == 2 && dpmax > 64) return launch_loglik_dp<4, FAM>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s);
+                                   const hipns_buffers_t& Bf, int K, int wpb, int dpmax, size_t lds, hipStream_t s,
+                                   const LinRegLds& lr) {
+  if (LPC == 2 && dpmax > 64) return launch_loglik_dp<4, FAM>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
   constexpr int CW = HIPNS_WAVE / LPC;
   int64_t groups = (B + CW - 1) / CW;
-  dim3 grid((unsigned)((groups + wpb - 1) / wpb)), block(wpb * HIPNS_WAVE);
+  dim3 grid((unsigned)(FAM == HIPNS_FAM_LINREG ? groups : (groups + wpb - 1) / wpb)), block(wpb * HIPNS_WAVE);
   if (dpmax <= 32)
-    hipLaunchKernelGGL((k_loglik<LPC, 32 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K);
+    hipLaunchKernelGGL((k_loglik<LPC, 32 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K, lr);
   else if (dpmax <= 64)
-    hipLaunchKernelGGL((k_loglik<LPC, 64 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K);
+    hipLaunchKernelGGL((k_loglik<LPC, 64 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K, lr);
   else if constexpr (LPC == 4)
-    hipLaunchKernelGGL((k_loglik<LPC, 128 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K);
+    hipLaunchKernelGGL((k_loglik<LPC, 128 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K, lr);
   return hipGetLastError();
 }
 
 hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
-                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s) {
+                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s,
+                         const LinRegLds& lr) {
+  const int lpc = L.lanes_per_chain, wpb = L.waves_per_block, K = L.gmm_K;
   if (L.family == HIPNS_FAM_GMM) {
-    if (L.lanes_per_chain == 4)
-      return launch_loglik_dp<4, HIPNS_FAM_GMM>(U, B, out, runs, Bf, L.gmm_K, L.waves_per_block, dpmax, lds, s);
-    return launch_loglik_dp<2, HIPNS_FAM_GMM>(U, B, out, runs, Bf, L.gmm_K, L.waves_per_block, dpmax, lds, s);
+    if (lpc == 4) return launch_loglik_dp<4, HIPNS_FAM_GMM>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
+    return launch_loglik_dp<2, HIPNS_FAM_GMM>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
   }
-  if (L.lanes_per_chain == 4)
-    return launch_loglik_dp<4, HIPNS_FAM_MVN>(U, B, out, runs, Bf, L.gmm_K, L.waves_per_block, dpmax, lds, s);
-  return launch_loglik_dp<2, HIPNS_FAM_MVN>(U, B, out, runs, Bf, L.gmm_K, L.waves_per_block, dpmax, lds, s);
+  if (L.family == HIPNS_FAM_LINREG) {
+    if (lpc == 4) return launch_loglik_dp<4, HIPNS_FAM_LINREG>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
+    return launch_loglik_dp<2, HIPNS_FAM_LINREG>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
+  }
+  if (lpc == 4) return launch_loglik_dp<4, HIPNS_FAM_MVN>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
+  return launch_loglik_dp<2, HIPNS_FAM_MVN>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
 }
 
 }  // namespace hipns

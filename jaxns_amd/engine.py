@@ -97,7 +97,7 @@ class DeviceBatch:
     """R independent runs sharing one likelihood family, advanced together."""
 
     def __init__(self, specs: List[RunSpec], *, lanes_per_chain: int = 2, waves_per_block: int = 4,
-                 max_steps: int = 512, device: str = "cuda"):
+                 max_steps: int = 512, device: str = "cuda", lr_terms: int = 3):
         torch = _torch()
         if not specs:
             raise ValueError("no runs")
@@ -130,6 +130,15 @@ class DeviceBatch:
             prms.append(p)
             consts.append(c)
         poffs = np.cumsum([0] + [p.size for p in prms])
+        lr_ndata = lr_xs = lr_off = 0
+        if self.family == _lib.FAM_LINREG:
+            data = fam.data
+            if any(s.model.log_likelihood.data is not data for s in specs):
+                raise ValueError("all regression runs of a batch must share one RegressionData")
+            blk, lr_ndata, lr_xs = data.packed(dps)
+            lr_off = int(-(-poffs[-1] // 64) * 64)  # 256-byte aligned start of the shared rows
+            prms.append(np.zeros(lr_off - poffs[-1], np.float32))
+            prms.append(blk)
         self.params = torch.from_numpy(np.concatenate(prms)).to(dev)
         # descriptors
         arr = (RunDesc * self.R)()
@@ -183,7 +192,10 @@ class DeviceBatch:
         b.params = ptr(self.params)
         b.dps = dps
         self.bufs = b
-        self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, 0, max_steps)
+        if lr_terms not in (3, 4):
+            raise ValueError("lr_terms must be 3 (BF16x3) or 4")
+        self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, lr_ndata, max_steps, lr_xs,
+                             lr_terms, lr_off)
         self._graph = None
         self._graph_iters = 0
         self._graph_term = None

@@ -47,3 +47,31 @@ def gmm_oracle_spec(D, means, sig):
     K = means.shape[0]
     return O.ModelSpec(O.FAM_GMM, D, O.PRIOR_UNIFORM, -10.0 * np.ones(D), 20.0 * np.ones(D), gmm_means=means,
                        gmm_sigma=sig, gmm_logw=np.log(np.full(K, 1.0 / K)))
+
+
+def linreg_problem(Ndata, D, seed=0, sigma=0.3, tau=1.0, scaled_cols=0.0, data=None):
+    """Bayesian linear regression: X ~ N(0,1), w_true ~ N(0,1), y = X w_true + sigma eps, prior w ~ N(0, tau^2 I).
+    scaled_cols > 0 multiplies that fraction of the columns of X by 100.  log Z = log N(y; 0, sigma^2 I + tau^2 X X^T)."""
+    if data is None:
+        rng = np.random.default_rng(seed)
+        X = rng.normal(size=(Ndata, D))
+        if scaled_cols > 0:
+            X[:, rng.permutation(D)[:max(1, int(round(scaled_cols * D)))]] *= 100.0
+        w = rng.normal(size=D)
+        y = X @ w + sigma * rng.normal(size=Ndata)
+        data = J.RegressionData(X, y)
+    D = data.D
+
+    def prior_model():
+        w = yield J.Prior(tfpd.Normal(loc=np.zeros(D), scale=tau * np.ones(D)), name="w")
+        return w
+
+    lik = J.LinearRegressionLikelihood(sigma=sigma, data=data)
+    return J.Model(prior_model, lik), lik.log_evidence_normal_prior(tau), lik
+
+
+def linreg_oracle_spec(lik, tau):
+    import oracle as O
+    D = lik.D
+    return O.ModelSpec(O.FAM_LINREG, D, O.PRIOR_NORMAL, np.zeros(D), tau * np.ones(D), lr_X=lik.X, lr_y=lik.y,
+                       lr_sigma=lik.sigma)

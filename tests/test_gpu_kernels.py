@@ -9,7 +9,8 @@ import oracle as O
 from jaxns_amd import _lib
 from jaxns_amd._lib import check, lib, ptr
 
-from problems import gmm_oracle_spec, gmm_problem, mvn_oracle_spec, mvn_problem
+from problems import (gmm_oracle_spec, gmm_problem, linreg_oracle_spec, linreg_problem, mvn_oracle_spec,
+                      mvn_problem)
 
 pytestmark = pytest.mark.gpu
 
@@ -136,59 +137,88 @@ def test_count_crossed_edges_vs_oracle(n, num_samples):
     assert np.array_equal(nl.cpu().numpy(), nl_o)
 
 
-def _batch(model, N=64, num_slices=None, k=0, seed=1, lpc=2, max_ms=None):
+def _batch(model, N=64, num_slices=None, k=0, seed=1, lpc=2, max_ms=None, wpb=2):
     from jaxns_amd.engine import DeviceBatch, RunSpec
     D = model.U_ndims
     spec = RunSpec(model=model, num_live_points=N, num_slices=num_slices or 2 * D, k_phantom=k,
                    max_samples=max_ms or N * 20, seed=seed)
-    return DeviceBatch([spec], lanes_per_chain=lpc, waves_per_block=2, max_steps=64)
+    return DeviceBatch([spec], lanes_per_chain=lpc, waves_per_block=wpb, max_steps=64)
 
 
-@pytest.mark.parametrize("fam,D,lpc", [("mvn", 10, 2), ("mvn", 32, 2), ("mvn", 32, 4), ("mvn", 70, 2),
-                                       ("gmm", 7, 2), ("gmm", 40, 4)])
-def test_batched_loglik_vs_oracle(fam, D, lpc):
-    """Model.forward on the device (FP32) vs the FP64 oracle: 1e-5 relative."""
+def _linreg_posterior_U(lik, tau, n, rng):
+    """U of n draws from the analytic posterior of w ~ N(0, tau^2 I)."""
+    from scipy.stats import norm
+    A = np.eye(lik.D) / tau ** 2 + lik.X.T @ lik.X / lik.sigma ** 2
+    m = np.linalg.solve(A, lik.X.T @ lik.y / lik.sigma ** 2)
+    W = rng.multivariate_normal(m, np.linalg.inv(A), size=n)
+    return norm.cdf(W / tau)
+
+
+@pytest.mark.parametrize("fam,D,lpc,wpb,nd", [("mvn", 10, 2, 2, 0), ("mvn", 32, 2, 2, 0), ("mvn", 32, 4, 2, 0),
+                                              ("mvn", 70, 2, 2, 0), ("gmm", 7, 2, 2, 0), ("gmm", 40, 4, 2, 0),
+                                              ("linreg", 16, 2, 4, 4096), ("linreg", 16, 4, 1, 4096),
+                                              ("linreg", 5, 2, 3, 37), ("linreg", 100, 4, 2, 1000)])
+def test_batched_loglik_vs_oracle(fam, D, lpc, wpb, nd):
+    """Model.forward on the device (FP32; BF16x3 MFMA for the regression rows) vs the FP64
+    oracle: 1e-5 relative."""
     torch = _torch()
-    say(f"loglik {fam} D={D} lpc={lpc}")
-    if fam == "mvn":
-        model, _, mu, Sig = mvn_problem(D, seed=D)
-        spec = mvn_oracle_spec(D, mu, Sig)
-    else:
-        model, _, means, sig = gmm_problem(D, seed=D)
-        spec = gmm_oracle_spec(D, means, sig)
-    b = _batch(model, lpc=lpc)
+    say(f"loglik {fam} D={D} lpc={lpc} wpb={wpb} ndata={nd}")
     rng = np.random.default_rng(0)
     B = 1000
     U = rng.uniform(0.02, 0.98, size=(B, D)).astype(np.float32)
-    if fam == "gmm":  # points near the modes so that the mixture is not -inf in FP32
+    if fam == "mvn":
+        model, _, mu, Sig = mvn_problem(D, seed=D)
+        spec = mvn_oracle_spec(D, mu, Sig)
+    elif fam == "gmm":
+        model, _, means, sig = gmm_problem(D, seed=D)
+        spec = gmm_oracle_spec(D, means, sig)
+        # points near the modes so that the mixture is not -inf in FP32
         U = ((means[rng.integers(0, means.shape[0], B)] + rng.normal(size=(B, D)) * 0.5 + 10) / 20).clip(0.01, 0.99)
         U = U.astype(np.float32)
+    else:
+        model, _, lik = linreg_problem(nd, D, seed=D, sigma=0.3, tau=1.0)
+        spec = linreg_oracle_spec(lik, 1.0)
+        U[B // 2:] = _linreg_posterior_U(lik, 1.0, B - B // 2, rng).clip(1e-6, 1 - 1e-6)
+    b = _batch(model, lpc=lpc, wpb=wpb)
     Up = np.full((B, b.dps), 0.5, np.float32)
     Up[:, :D] = U
     Ud = torch.from_numpy(Up).cuda()
-    out = torch.empty(B, dtype=torch.float64, device="cuda")
-    check(lib().hipns_log_likelihood(ptr(Ud), B, ptr(out), ptr(b.runs_dev), b.runs_host, C.byref(b.bufs),
-                                     C.byref(b.launch), _stream()))
-    got = out.cpu().numpy()
     ref = spec.log_likelihood(U.astype(np.float64))
-    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
-    say("max rel err", rel.max())
-    assert rel.max() < 1e-5
+    for terms in ((3, 4) if fam == "linreg" else (0,)):
+        out = torch.empty(B, dtype=torch.float64, device="cuda")
+        b.launch.lr_terms = terms
+        check(lib().hipns_log_likelihood(ptr(Ud), B, ptr(out), ptr(b.runs_dev), b.runs_host, C.byref(b.bufs),
+                                         C.byref(b.launch), _stream()))
+        got = out.cpu().numpy()
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
+        say("terms", terms, "max rel err", rel.max())
+        # BF16x3 on prior draws: the 16-bit split of w - w0 is shared by all rows, 3e-5 relative;
+        # every other form: 1e-5 relative
+        assert rel.max() < (3e-5 if terms == 3 else 1e-5)
+        if fam == "linreg":  # posterior draws: the centred residual keeps the error far below the posterior scale
+            post = np.abs(got - ref)[B // 2:]
+            say("posterior draws: max abs err", post.max())
+            assert post.max() < 1e-6 * np.abs(ref[B // 2:]).max() + 2e-3
 
 
-@pytest.mark.parametrize("D,lpc,k", [(10, 2, 0), (32, 2, 2), (32, 4, 0), (5, 4, 1)])
-def test_slice_chains_injected_bit_exact(D, lpc, k):
+@pytest.mark.parametrize("fam,D,lpc,k,wpb", [("mvn", 10, 2, 0, 2), ("mvn", 32, 2, 2, 2), ("mvn", 32, 4, 0, 2),
+                                             ("mvn", 5, 4, 1, 2), ("linreg", 16, 2, 1, 4), ("linreg", 40, 4, 0, 3)])
+def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb):
     """UniDimSliceSampler chains with injected draws: the device trace (t of every pick),
     the accept/reject sequence, eval counts and the accepted points are bit-exact with
     the binary32 oracle replaying the device's likelihood values; the device likelihood
     itself agrees with the FP64 oracle to 1e-5."""
     torch = _torch()
-    say(f"slice injected D={D} lpc={lpc} k={k}")
-    model, _, mu, Sig = mvn_problem(D, seed=100 + D)
-    spec = mvn_oracle_spec(D, mu, Sig)
+    say(f"slice injected {fam} D={D} lpc={lpc} k={k} wpb={wpb}")
+    if fam == "mvn":
+        model, _, mu, Sig = mvn_problem(D, seed=100 + D)
+        spec = mvn_oracle_spec(D, mu, Sig)
+    else:
+        model, _, lik = linreg_problem(300, D, seed=100 + D, sigma=0.5, tau=1.0)
+        spec = linreg_oracle_spec(lik, 1.0)
     N = 64
     ns = 2 * D
-    b = _batch(model, N=N, num_slices=ns, k=k, seed=7, lpc=lpc)
+    b = _batch(model, N=N, num_slices=ns, k=k, seed=7, lpc=lpc, wpb=wpb)
     b.init_state()
     torch.cuda.synchronize()
     live_U = b.live_U[0][:N, :D].cpu().numpy()
@@ -211,7 +241,8 @@ def test_slice_chains_injected_bit_exact(D, lpc, k):
     kk = max(k, 1)
     phU = torch.zeros((Cn, kk, b.dps), dtype=torch.float32, device="cuda")
     phL = torch.zeros((Cn, kk), dtype=torch.float64, device="cuda")
-    launch = _lib.Launch(_lib.FAM_MVN, lpc, 2, 1, 0, max_steps)
+    launch = _lib.Launch.from_buffer_copy(b.launch)
+    launch.max_steps = max_steps
     check(lib().hipns_slice_chains_injected(ptr(b.runs_dev), b.runs_host, C.byref(b.bufs), C.byref(launch), 0, Cn,
                                             contour, ptr(sd), ptr(dd), ptr(td), ptr(tr_t), ptr(tr_L), ptr(steps),
                                             ptr(phU), ptr(phL), _stream()))
@@ -273,3 +304,70 @@ def test_batched_runs_match_single_runs():
     assert np.array_equal(a.log_L_samples, b.log_L_samples)
     assert np.array_equal(a.U_samples, b.U_samples)
     assert a.log_Z_mean == b.log_Z_mean
+
+
+def test_uneven_batched_gmm_runs_logZ():
+    """Runs of uneven dimension and live-point count (mixture likelihood, uniform box prior)
+    advanced in one batch: each run's log Z against its own analytic value."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    say("uneven GMM batch")
+    rng = np.random.default_rng(123)
+    specs, truths = [], []
+    for r in range(8):
+        D = int(rng.integers(2, 9))
+        N = int(rng.integers(128, 320))
+        model, log_Z, _, _ = gmm_problem(D, seed=1000 + r)
+        specs.append(RunSpec(model=model, num_live_points=N, num_slices=5 * D, max_samples=N * 400, seed=r))
+        truths.append(log_Z)
+    b = DeviceBatch(specs, lanes_per_chain=4, waves_per_block=2)
+    b.run(dict(dlogZ=math.log(1.0 + 1e-3), max_samples=1e18), graph_iters=4)
+    z = b.log_Z()
+    for r in range(8):
+        say(r, specs[r].model.U_ndims, specs[r].num_live_points, z[r], truths[r])
+        assert abs(z[r, 0] - truths[r]) < 4 * z[r, 1] + 0.25
+    sts = b.states()
+    assert all(st.termination_reason & 4 for st in sts)
+
+
+def test_linreg_hyperparameter_batch_logZ():
+    """Bayesian linear regression runs at four (sigma, tau) settings sharing one data set in one
+    batch (split-row BF16x3 likelihood): each log Z against log N(y; 0, sigma^2 I + tau^2 X X^T)."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    say("linreg hyperparameter batch")
+    _, _, lik0 = linreg_problem(1024, 8, seed=5, sigma=0.3)
+    specs, truths = [], []
+    for r, (sg, tau) in enumerate([(0.3, 1.0), (0.5, 1.0), (0.3, 2.0), (0.2, 0.5)]):
+        model, log_Z, _ = linreg_problem(0, 0, sigma=sg, tau=tau, data=lik0.data)
+        specs.append(RunSpec(model=model, num_live_points=256, num_slices=4 * 8, max_samples=256 * 400, seed=r))
+        truths.append(log_Z)
+    b = DeviceBatch(specs, lanes_per_chain=2, waves_per_block=4)
+    b.run(dict(dlogZ=math.log(1.0 + 1e-3), max_samples=1e18), graph_iters=4)
+    z = b.log_Z()
+    for r in range(4):
+        say(r, z[r], truths[r])
+        assert abs(z[r, 0] - truths[r]) < 4 * z[r, 1] + 0.2
+    assert all(st.termination_reason & 4 for st in b.states())
+
+
+def test_uniform_sampler_phase():
+    """UniformSampler shells (samplers/uniform_samplers.py:42-85): every new sample satisfies
+    the acceptance rule (logL > contour, or the cap of 100 likelihood evaluations)."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    say("uniform sampler shells")
+    model, _, _, _ = mvn_problem(3, seed=2)
+    spec = RunSpec(model=model, num_live_points=200, num_slices=15, max_samples=200 * 50, seed=5)
+    b = DeviceBatch([spec], lanes_per_chain=4, waves_per_block=2)
+    b.init_state()
+    b.set_sampler(1)
+    term = dict(max_samples=1e18)
+    for it in range(6):
+        b.collect_shell(term)
+        st = b.states()[0]
+        import torch
+        torch.cuda.synchronize()
+        ev = b.new_evals[:100].cpu().numpy()
+        L = b.new_logL[:100].cpu().numpy()
+        ok = (L > st.contour) | (L == st.contour) | (ev >= 100)
+        assert ok.all(), it
+        assert np.all(ev >= 1) and np.all(ev <= 100)
+    b.set_sampler(0)

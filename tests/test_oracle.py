@@ -15,7 +15,7 @@ from scipy.special import ndtri
 
 import oracle as O
 
-from problems import mvn_oracle_spec
+from problems import linreg_oracle_spec, linreg_problem, mvn_oracle_spec
 
 
 # ---------------------------------------------------------------- tree counts
@@ -275,3 +275,15 @@ def test_oracle_full_run_analytic():
         assert r["termination_reason"] & 4
     zs = np.array(zs)
     assert abs(zs[:, 0].mean() - log_Z) < 3 * zs[:, 1].mean() / 2 + 0.05
+
+
+def test_linreg_loglik_vs_scipy():
+    """Oracle regression likelihood at x(u) = tau ndtri(u) vs scipy's normal logpdf summed over rows."""
+    _, _, lik = linreg_problem(200, 7, seed=3, sigma=0.5, tau=2.0)
+    spec = linreg_oracle_spec(lik, 2.0)
+    rng = np.random.default_rng(0)
+    U = rng.uniform(0.01, 0.99, size=(10, 7))
+    got = spec.log_likelihood(U)
+    W = 2.0 * ndtri(U)
+    want = np.array([stats.norm.logpdf(lik.y, lik.X @ w, 0.5).sum() for w in W])
+    np.testing.assert_allclose(got, want, rtol=1e-12)
