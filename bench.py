@@ -198,7 +198,8 @@ def instrumented_slice_timing(wl, R, lpc, wpb, seed0):
     b.check_termination(term, True)
     tc = make_term_cond(**term)
     wpe = np.array([wl.work_per_eval(sp) for sp in b.specs])
-    durs, work, evals = [], [], []
+    durs, work, evals, util = [], [], [], []
+    CW = 64 // (4 if b.dps > 64 else lpc)
     s = stream_ptr()
     prev = np.array([st.num_likelihood_evaluations for st in b.states()], dtype=np.float64)
     while not np.all(b.done_flags()):
@@ -213,10 +214,24 @@ def instrumented_slice_timing(wl, R, lpc, wpb, seed0):
                                          C.byref(tc), s))
         torch.cuda.synchronize()
         cur = np.array([st.num_likelihood_evaluations for st in b.states()], dtype=np.float64)
+        # lane utilisation of the launch: chain steps over wavefront steps (a wavefront runs until
+        # its slowest chain finishes)
+        ne = b.new_evals.cpu().numpy().astype(np.float64)
+        busy = tot = 0.0
+        for r, sp in enumerate(b.specs):
+            if cur[r] == prev[r]:
+                continue
+            e = ne[b.runs_host[r].chain_off:b.runs_host[r].chain_off + sp.shell]
+            pad = (-e.size) % CW
+            e = np.concatenate([e, np.zeros(pad)]).reshape(-1, CW)
+            busy += e.sum()
+            tot += e.max(axis=1).sum() * CW
+        util.append(busy / max(tot, 1.0))
         durs.append(e0.elapsed_time(e1) * 1e-3)
         work.append(float(((cur - prev) * wpe).sum()))
         evals.append(float((cur - prev).sum()))
         prev = cur
+    instrumented_slice_timing.lane_util = float(np.mean(util)) if util else float("nan")
     return np.array(durs), np.array(work), np.array(evals)
 
 
@@ -294,7 +309,8 @@ def main():
         print(json.dumps({"profile_pass": True, "config": args.config, "runs": R, "lanes_per_chain": args.lpc,
                           "waves_per_block": args.wpb, "sample_launches": int(durs.size),
                           "avg_launch_ms": float(durs.mean() * 1e3), "evals_per_launch": float(evs.mean()),
-                          "achieved": float(work.mean() / durs.mean()), "unit": wl.unit[:-2] + "/s"}), flush=True)
+                          "achieved": float(work.mean() / durs.mean()), "unit": wl.unit[:-2] + "/s",
+                          "lane_utilisation": instrumented_slice_timing.lane_util}), flush=True)
         return
     if not args.no_autotune:
         from jaxns_amd.autotune import autotune
@@ -361,7 +377,8 @@ def main():
     roofline = dict(bound=wl.bound, achieved=achieved / scale, peak=wl.peak / scale, unit=wl.unit,
                     frac=achieved / wl.peak, traffic=None, kernel=wl.kernel, avg_launch_ms=avg_dur * 1e3,
                     launches=int(durs.size), evals_per_launch=float(evs.mean()),
-                    evals_per_s_in_kernel=float(evs.sum() / durs.sum()))
+                    evals_per_s_in_kernel=float(evs.sum() / durs.sum()),
+                    lane_utilisation=instrumented_slice_timing.lane_util)
 
     cpu = None
     if not args.no_cpu:

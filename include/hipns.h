@@ -134,7 +134,7 @@ typedef struct hipns_launch {
   int32_t max_steps;        /* step cap per slice (safety bound on the shrink loop) */
   int32_t lr_xstride;       /* LinReg: bf16 elements per X row (a multiple of 32) */
   int32_t lr_terms;         /* LinReg: 3 (0) = BF16x3 hi.hi + hi.lo + lo.hi; 4 adds X hi . (third 8 bits of w) */
-  int64_t lr_data_off;      /* LinReg: float offset in params of [r0 (lr_ndata) | X hi | X lo] shared by all runs */
+  int64_t lr_data_off;      /* LinReg: float offset in params of [r0 | X hi | X lo | y] shared by all runs */
 } hipns_launch_t;
 
 /* ---------------------------------------------------------------------
@@ -223,6 +223,17 @@ int hipns_count_crossed_edges(const int32_t* sender_node_idx_dev, const double* 
                               void* workspace, size_t workspace_bytes, void* stream);
 size_t hipns_count_crossed_edges_workspace_size(int64_t N);
 
+/* The evidence statistics of _to_results (sharded_static.py:652-745: count_crossed_edges, then
+ * compute_evidence_stats over the sorted samples) for R runs at once, in one set of launches.
+ * Run r's samples are its sample-collection slice of M_r = min(num_samples, max_samples) entries,
+ * read from states_dev on the device.  final_dev [R][8] receives the final EvidenceCalculation
+ * (log_L, log_X_mean, log_X2_mean, log_Z_mean, log_ZX_mean, log_Z2_mean, log_dZ_mean,
+ * log_dZ2_mean), phantom_dev [R] the number of phantom samples. */
+int hipns_batch_evidence(const hipns_run_desc_t* runs_dev, const hipns_run_state_t* states_dev, int32_t R,
+                         const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, double* final_dev,
+                         int64_t* phantom_dev, void* workspace, size_t workspace_bytes, void* stream);
+size_t hipns_batch_evidence_workspace_size(int32_t R, const hipns_run_desc_t* runs_host);
+
 /* Stable argsort of FP64 keys (jnp.argsort, stable=True; -inf < ... < +inf < NaN). */
 int hipns_argsort_stable(const double* keys_dev, int64_t n, int64_t* perm_dev, void* workspace,
                          size_t workspace_bytes, void* stream);
@@ -238,6 +249,16 @@ size_t hipns_argsort_workspace_size(int64_t n);
 int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const hipns_run_desc_t* runs_dev,
                          const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
                          void* stream);
+
+/* Adjoint statistics of the regression likelihood, for the M-step of evidence maximisation
+ * (experimental/evidence_maximisation.py:151-160, log_evidence = logsumexp(forward(U; theta) +
+ * log_weights), differentiated by jax there).  For each row b of U (stride bufs->dps), with
+ * w = p + s (.) q(U_b) from run 0's prior and r = y - X w:  logL_dev[b], rss_dev[b] = |r|^2,
+ * yr_dev[b] = y^T r.  Then d logL / d log sigma = -N + |r|^2 / sigma^2 and, for a zero-mean prior
+ * scaled by tau, d logL / d log tau = w^T X^T r / sigma^2 = (y^T r - |r|^2) / sigma^2. */
+int hipns_linreg_stats(const float* U_dev, int64_t B, double* logL_dev, double* rss_dev, double* yr_dev,
+                       const hipns_run_desc_t* runs_dev, const hipns_run_desc_t* runs_host,
+                       const hipns_buffers_t* bufs, const hipns_launch_t* launch, void* stream);
 
 /* Prior.forward of every dimension (framework/bases.py:58-68 ->
  * framework/wrapped_tfp_distribution.py:77-84): X = loc + scale (.) q(U) for

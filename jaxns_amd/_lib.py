@@ -120,6 +120,10 @@ def _declare(L):
     L.hipns_argsort_workspace_size.restype = C.c_size_t
     L.hipns_argsort_workspace_size.argtypes = [_i64]
     L.hipns_log_likelihood.argtypes = [_vp, _i64, _vp, _vp, P(RunDesc), P(Buffers), P(Launch), _vp]
+    L.hipns_batch_evidence.argtypes = [_vp, _vp, _i32, P(RunDesc), P(Buffers), _vp, _vp, _vp, C.c_size_t, _vp]
+    L.hipns_batch_evidence_workspace_size.restype = C.c_size_t
+    L.hipns_batch_evidence_workspace_size.argtypes = [_i32, P(RunDesc)]
+    L.hipns_linreg_stats.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, P(RunDesc), P(Buffers), P(Launch), _vp]
     L.hipns_prior_transform.argtypes = [_vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp]
     L.hipns_slice_chains_injected.argtypes = [_vp, P(RunDesc), P(Buffers), P(Launch), _i32, _i32, _d, _vp, _vp,
                                               _vp, _vp, _vp, _vp, _vp, _vp, _vp]
@@ -132,7 +136,8 @@ EXPORTED = ["hipns_abi_version", "hipns_last_error", "hipns_create_init_state", 
             "hipns_consume_live", "hipns_check_termination", "hipns_collect_shell_workspace_size",
             "hipns_evidence_scan", "hipns_evidence_scan_workspace_size", "hipns_count_crossed_edges",
             "hipns_count_crossed_edges_workspace_size", "hipns_argsort_stable", "hipns_argsort_workspace_size",
-            "hipns_log_likelihood", "hipns_prior_transform", "hipns_slice_chains_injected", "hipns_philox4x32",
+            "hipns_log_likelihood", "hipns_linreg_stats", "hipns_batch_evidence",
+            "hipns_batch_evidence_workspace_size", "hipns_prior_transform", "hipns_slice_chains_injected", "hipns_philox4x32",
             "hipns_ndtri_f32"]
 
 

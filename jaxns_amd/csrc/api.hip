@@ -86,6 +86,22 @@ int check_launch(const hipns_launch_t* L) {
   return HIPNS_OK;
 }
 
+// device view of the shared regression rows: params + lr_data_off = [r0: npad][X hi: npad x xs bf16]
+// [X lo: npad x xs bf16][y: npad]
+hipns::LinRegLds linreg_view(const hipns_buffers_t* bufs, const hipns_launch_t* L) {
+  hipns::LinRegLds lr{};
+  const float* base = bufs->params + L->lr_data_off;
+  const int64_t npad = L->lr_ndata, xs = L->lr_xstride;
+  lr.r0 = base;
+  lr.xh = (const uint16_t*)(base + npad);
+  lr.xl = lr.xh + npad * xs;
+  lr.y = base + npad + npad * xs;
+  lr.npad = L->lr_ndata;
+  lr.xs = L->lr_xstride;
+  lr.terms = L->lr_terms == 4 ? 4 : 3;
+  return lr;
+}
+
 int max_lds_bytes() {
   int dev = 0, v = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 65536;
@@ -99,13 +115,13 @@ int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A
   hipns::SampleArgs A = A0;
   if (L->family == HIPNS_FAM_LINREG) {
     if (L->lr_xstride < dps) return fail(HIPNS_EINVAL, "lr_xstride %d below dps %d", L->lr_xstride, dps);
-    const float* base = A.B.params + L->lr_data_off;
-    A.lr_r0 = base;
-    A.lr_xh = (const uint16_t*)(base + L->lr_ndata);
-    A.lr_xl = A.lr_xh + (int64_t)L->lr_ndata * L->lr_xstride;
-    A.lr_npad = L->lr_ndata;
-    A.lr_xs = L->lr_xstride;
-    A.lr_terms = L->lr_terms == 4 ? 4 : 3;
+    const hipns::LinRegLds lr = linreg_view(&A.B, L);
+    A.lr_r0 = lr.r0;
+    A.lr_xh = lr.xh;
+    A.lr_xl = lr.xl;
+    A.lr_npad = lr.npad;
+    A.lr_xs = lr.xs;
+    A.lr_terms = lr.terms;
   }
   A.alpha_off = (int32_t)hipns::sampler_lds_floats(L->family, dps, maxD, L->gmm_K, lpc, L->waves_per_block);
   size_t lds = ((size_t)A.alpha_off + (size_t)max_slices) * sizeof(float);
@@ -351,6 +367,32 @@ int hipns_evidence_scan(const double* log_L_dev, const double* num_live_points_d
 
 size_t hipns_count_crossed_edges_workspace_size(int64_t N) { return hipns::count_crossed_edges_ws(N); }
 
+size_t hipns_batch_evidence_workspace_size(int32_t R, const hipns_run_desc_t* runs_host) {
+  if (R <= 0 || !runs_host) return 0;
+  int64_t mx = 0;
+  for (int r = 0; r < R; ++r) mx = runs_host[r].max_samples > mx ? runs_host[r].max_samples : mx;
+  return hipns::batch_evidence_ws(R, mx);
+}
+
+int hipns_batch_evidence(const hipns_run_desc_t* runs_dev, const hipns_run_state_t* states_dev, int32_t R,
+                         const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, double* final_dev,
+                         int64_t* phantom_dev, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc, maxD, maxN, maxS;
+  if ((rc = check_buffers(bufs))) return rc;
+  if ((rc = check_runs(runs_host, R, bufs, &maxD, &maxN, &maxS))) return rc;
+  if (R > 65535) return fail(HIPNS_ERANGE, "at most 65535 runs per call");
+  if (!runs_dev || !states_dev || !final_dev || !phantom_dev) return fail(HIPNS_EINVAL, "null pointer");
+  const size_t need = hipns_batch_evidence_workspace_size(R, runs_host);
+  if (!workspace || workspace_bytes < need)
+    return fail(HIPNS_ENOMEM, "workspace of %zu B needed, %zu given", need, workspace_bytes);
+  int64_t mx = 0;
+  for (int r = 0; r < R; ++r) mx = runs_host[r].max_samples > mx ? runs_host[r].max_samples : mx;
+  hipError_t e = hipns::batch_evidence(runs_dev, states_dev, R, mx, *bufs, final_dev, phantom_dev, workspace,
+                                       as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "batch evidence");
+  return HIPNS_OK;
+}
+
 int hipns_count_crossed_edges(const int32_t* sender_node_idx_dev, const double* log_L_dev, int64_t N,
                               int64_t num_samples, int64_t* samples_indices_dev, int64_t* num_live_points_dev,
                               void* workspace, size_t workspace_bytes, void* stream) {
@@ -406,16 +448,32 @@ int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const 
   hipns::LinRegLds lr{};
   if (launch->family == HIPNS_FAM_LINREG) {
     if (launch->lr_xstride < bufs->dps) return fail(HIPNS_EINVAL, "lr_xstride below dps");
-    const float* base = bufs->params + launch->lr_data_off;
-    lr.r0 = base;
-    lr.xh = (const uint16_t*)(base + launch->lr_ndata);
-    lr.xl = lr.xh + (int64_t)launch->lr_ndata * launch->lr_xstride;
-    lr.npad = launch->lr_ndata;
-    lr.xs = launch->lr_xstride;
-    lr.terms = launch->lr_terms == 4 ? 4 : 3;
+    lr = linreg_view(bufs, launch);
   }
   hipError_t e = hipns::launch_loglik(U_dev, B, logL_dev, runs_dev, *bufs, L2, bufs->dps, lds, as_stream(stream), lr);
   if (e != hipSuccess) return hip_fail(e, "loglik launch");
+  return HIPNS_OK;
+}
+
+int hipns_linreg_stats(const float* U_dev, int64_t B, double* logL_dev, double* rss_dev, double* yr_dev,
+                       const hipns_run_desc_t* runs_dev, const hipns_run_desc_t* runs_host,
+                       const hipns_buffers_t* bufs, const hipns_launch_t* launch, void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  if ((rc = check_launch(launch))) return rc;
+  if (launch->family != HIPNS_FAM_LINREG) return fail(HIPNS_EINVAL, "hipns_linreg_stats needs the regression family");
+  if (!runs_host || !runs_dev) return fail(HIPNS_EINVAL, "null runs");
+  if (B < 0) return fail(HIPNS_EINVAL, "B < 0");
+  if (B == 0) return HIPNS_OK;
+  if (!U_dev || !logL_dev || !rss_dev || !yr_dev) return fail(HIPNS_EINVAL, "null pointer");
+  const int D = runs_host[0].D;
+  if (D < 1 || D > bufs->dps) return fail(HIPNS_EINVAL, "D outside [1, dps]");
+  if (launch->lr_xstride < bufs->dps) return fail(HIPNS_EINVAL, "lr_xstride below dps");
+  const int lpc = bufs->dps > 64 ? 4 : launch->lanes_per_chain;
+  const size_t lds = hipns::sampler_lds_floats(HIPNS_FAM_LINREG, bufs->dps, D, 1, lpc, 1) * sizeof(float);
+  hipError_t e = hipns::launch_linreg_stats(U_dev, B, logL_dev, rss_dev, yr_dev, runs_dev, *bufs, lpc, bufs->dps, lds,
+                                            as_stream(stream), linreg_view(bufs, launch));
+  if (e != hipSuccess) return hip_fail(e, "linreg stats launch");
   return HIPNS_OK;
 }
 

@@ -1,5 +1,7 @@
 // ns_results.hip -- to_results primitives: the evidence scan over all samples,
 // a stable FP64 argsort, and count_crossed_edges.
+#include <utility>
+
 #include "hipns_dev.h"
 #include "ns_shell.h"
 
@@ -50,15 +52,32 @@ __global__ void k_fill_zero_u32(uint32_t* p, int64_t n) {
   if (i < n) p[i] = 0u;
 }
 
-__global__ void __launch_bounds__(EV_THREADS) k_evidence_scan(const double* logL, const double* nlive, int64_t M,
-                                                              int64_t stop, double* per8, double* fin8,
-                                                              TileStatus* status, uint32_t* tile_counter) {
+// sources of the scan: (logL, n) at sorted position i
+struct EvSrcFlat {
+  const double* logL;
+  const double* n;
+  __device__ double L(int64_t i) const { return logL[i]; }
+  __device__ double N(int64_t i) const { return n[i]; }
+};
+// sorted through a node permutation: position i holds sample perm[i + 1] - 1 (node 0 is the root)
+struct EvSrcSorted {
+  const double* logL;
+  const int32_t* perm;
+  const int64_t* nl;
+  __device__ double L(int64_t i) const { return logL[perm[i + 1] - 1]; }
+  __device__ double N(int64_t i) const { return (double)nl[i]; }
+};
+
+template <class Src>
+__device__ __forceinline__ void evidence_scan_tile(const Src& src, int64_t M, int64_t stop, double* per8, double* fin8,
+                                                   TileStatus* status, uint32_t* tile_counter) {
   __shared__ EvMap wave_aggs[EV_THREADS / 64];
   __shared__ EvMap s_excl;
   __shared__ uint32_t s_tile;
   if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
   __syncthreads();
   const int64_t tile = s_tile;
+  if (tile * EV_TILE >= M) return;  // block-uniform: no tile left for this block
   const int tid = threadIdx.x;
   const int l = lane_id(), w = tid / 64;
   const int64_t base = tile * EV_TILE + (int64_t)tid * EV_ITEMS;
@@ -68,17 +87,20 @@ __global__ void __launch_bounds__(EV_THREADS) k_evidence_scan(const double* logL
   EvMap m = evmap_identity();
   EvMap item[EV_ITEMS];
   double item_lt[EV_ITEMS];
+  double item_L[EV_ITEMS];
 #pragma unroll
   for (int k = 0; k < EV_ITEMS; ++k) {
     int64_t i = base + k;
     EvMap f = evmap_identity();
-    double lt = ninf;
+    double lt = ninf, Li = ninf;
     if (i < M && i < stop) {
-      double Lp = i == 0 ? ninf : logL[i - 1];
-      f = evmap_step(nlive[i], Lp, logL[i], &lt);
+      double Lp = i == 0 ? ninf : src.L(i - 1);
+      Li = src.L(i);
+      f = evmap_step(src.N(i), Lp, Li, &lt);
     }
     item[k] = f;
     item_lt[k] = lt;
+    item_L[k] = Li;
     m = evmap_then(m, f);
   }
   // inclusive scan across the wavefront (ordered)
@@ -134,43 +156,49 @@ __global__ void __launch_bounds__(EV_THREADS) k_evidence_scan(const double* logL
   EvState s;
   const EvState s_init = {ninf, 0.0, 0.0, ninf, ninf, ninf, ninf, ninf};
   s = evmap_apply(pre, s_init);
-  s.L = (base == 0) ? ninf : (base - 1 < stop ? logL[base - 1] : ninf);
+  s.L = (base == 0) ? ninf : (base - 1 < stop ? src.L(base - 1) : ninf);
   s.dZ = ninf;
 #pragma unroll
   for (int k = 0; k < EV_ITEMS; ++k) {
     int64_t i = base + k;
     if (i < M) {
-    EvState o;
-    if (i < stop) {
-      o = evmap_apply(item[k], s);
-      o.L = logL[i];
-      o.dZ = item_lt[k] + s.X;
-      s = o;
-    } else {
-      o = s_init;
-    }
-    if (per8) {
-      per8[0 * M + i] = o.L;
-      per8[1 * M + i] = o.X;
-      per8[2 * M + i] = o.X2;
-      per8[3 * M + i] = o.Z;
-      per8[4 * M + i] = o.ZX;
-      per8[5 * M + i] = o.Z2;
-      per8[6 * M + i] = o.dZ;
-      per8[7 * M + i] = o.dZ2;
-    }
-    if (fin8 && i == (stop < M ? stop : M) - 1) {
-      fin8[0] = o.L;
-      fin8[1] = o.X;
-      fin8[2] = o.X2;
-      fin8[3] = o.Z;
-      fin8[4] = o.ZX;
-      fin8[5] = o.Z2;
-      fin8[6] = o.dZ;
-      fin8[7] = o.dZ2;
-    }
+      EvState o;
+      if (i < stop) {
+        o = evmap_apply(item[k], s);
+        o.L = item_L[k];
+        o.dZ = item_lt[k] + s.X;
+        s = o;
+      } else {
+        o = s_init;
+      }
+      if (per8) {
+        per8[0 * M + i] = o.L;
+        per8[1 * M + i] = o.X;
+        per8[2 * M + i] = o.X2;
+        per8[3 * M + i] = o.Z;
+        per8[4 * M + i] = o.ZX;
+        per8[5 * M + i] = o.Z2;
+        per8[6 * M + i] = o.dZ;
+        per8[7 * M + i] = o.dZ2;
+      }
+      if (fin8 && i == (stop < M ? stop : M) - 1) {
+        fin8[0] = o.L;
+        fin8[1] = o.X;
+        fin8[2] = o.X2;
+        fin8[3] = o.Z;
+        fin8[4] = o.ZX;
+        fin8[5] = o.Z2;
+        fin8[6] = o.dZ;
+        fin8[7] = o.dZ2;
+      }
     }
   }
+}
+
+__global__ void __launch_bounds__(EV_THREADS) k_evidence_scan(const double* logL, const double* nlive, int64_t M,
+                                                              int64_t stop, double* per8, double* fin8,
+                                                              TileStatus* status, uint32_t* tile_counter) {
+  evidence_scan_tile(EvSrcFlat{logL, nlive}, M, stop, per8, fin8, status, tile_counter);
 }
 
 __global__ void k_fin_init(double* fin8) {
@@ -450,6 +478,320 @@ hipError_t count_crossed_edges(const int32_t* sender, const double* logL, int64_
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tsums, tiles);
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, deg, perm, n, stop, fake, tsums,
                      num_live, samples_indices);
+  return hipGetLastError();
+}
+
+// ===========================================================================
+// _to_results evidence of R runs at once: count_crossed_edges and the evidence
+// scan of every run of a batch in one set of launches (grid.y = run).  Run r's
+// samples are samp_*[samp_off_r, samp_off_r + M_r) with M_r = min(num_samples,
+// max_samples) read on the device; per-run segments of the workspace have a
+// common stride S >= max_r(M_r) + 1.
+// ===========================================================================
+__device__ __forceinline__ int64_t batch_M(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int r) {
+  const int64_t ns = states[r].num_samples, ms = runs[r].max_samples;
+  return ns < ms ? ns : ms;
+}
+
+__global__ void k_b_nodes(const hipns_run_desc_t* runs, const hipns_run_state_t* states, const double* samp_logL,
+                          int64_t S, double* nodes) {
+  const int r = blockIdx.y;
+  const int64_t M = batch_M(runs, states, r);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > M) return;
+  nodes[r * S + i] = i == 0 ? -__builtin_inf() : samp_logL[runs[r].samp_off + i - 1];
+}
+
+__global__ void __launch_bounds__(SORT_THREADS) k_b_tile_sort(const hipns_run_desc_t* runs,
+                                                              const hipns_run_state_t* states, const double* keys_in,
+                                                              int64_t S, double* keys_out, int32_t* idx_out) {
+  __shared__ double k[SORT_TILE];
+  __shared__ int32_t v[SORT_TILE];
+  const int r = blockIdx.y;
+  const int64_t n = batch_M(runs, states, r) + 1;
+  const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+  if (base >= n) return;  // block-uniform
+  const double* kin = keys_in + r * S;
+  for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
+    int64_t g = base + i;
+    k[i] = g < n ? kin[g] : __builtin_nan("");
+    v[i] = g < n ? (int32_t)g : (int32_t)0x7fffffff;
+  }
+  __syncthreads();
+  for (int kk = 2; kk <= SORT_TILE; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
+        int p = i ^ j;
+        if (p > i) {
+          bool up = (i & kk) == 0;
+          double ki = k[i], kp = k[p];
+          int32_t ii = v[i], ip = v[p];
+          bool sw = up ? key_lt(kp, ip, ki, ii) : key_lt(ki, ii, kp, ip);
+          if (sw) {
+            k[i] = kp;
+            k[p] = ki;
+            v[i] = ip;
+            v[p] = ii;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
+    int64_t g = base + i;
+    if (g < n) {
+      keys_out[r * S + g] = k[i];
+      idx_out[r * S + g] = v[i];
+    }
+  }
+}
+
+__global__ void k_b_merge_pass(const hipns_run_desc_t* runs, const hipns_run_state_t* states, const double* kin_all,
+                               const int32_t* vin_all, int64_t S, int64_t run, double* kout_all, int32_t* vout_all) {
+  const int r = blockIdx.y;
+  const int64_t n = batch_M(runs, states, r) + 1;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const double* kin = kin_all + r * S;
+  int64_t pair = g / (2 * run);
+  int64_t lbeg = pair * 2 * run;
+  int64_t rbeg = lbeg + run;
+  int64_t lend = rbeg < n ? rbeg : n;
+  int64_t rend = rbeg + run < n ? rbeg + run : n;
+  double key = kin[g];
+  int64_t pos;
+  if (g < rbeg) {
+    int64_t lo = rbeg, hi = rend;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (kless(kin[mid], key)) lo = mid + 1;
+      else hi = mid;
+    }
+    pos = lbeg + (g - lbeg) + (lo - rbeg);
+  } else {
+    int64_t lo = lbeg, hi = lend;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (!kless(key, kin[mid])) lo = mid + 1;
+      else hi = mid;
+    }
+    pos = lbeg + (g - rbeg) + (lo - lbeg);
+  }
+  kout_all[r * S + pos] = key;
+  vout_all[r * S + pos] = vin_all[r * S + g];
+}
+
+// out-degree of every node by sender (tree_structure.py:33-108) and the phantom count of each run
+__global__ void __launch_bounds__(256) k_b_out_degree(const hipns_run_desc_t* runs, const hipns_run_state_t* states,
+                                                      hipns_buffers_t B, int64_t S, int32_t* deg,
+                                                      unsigned long long* phantom) {
+  __shared__ unsigned long long red[4];
+  const int r = blockIdx.y;
+  const int64_t M = batch_M(runs, states, r);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t off = runs[r].samp_off;
+  unsigned long long ph = 0;
+  if (i < M) {
+    int32_t snd = B.samp_sender[off + i];
+    int64_t sn = snd > 0 ? snd : 0;                // lax.max(sender, 0)
+    if (sn <= M) atomicAdd(&deg[r * S + sn], 1);  // out-of-range scatter updates are dropped
+    ph = B.samp_phantom[off + i] ? 1ull : 0ull;
+  }
+  for (int o = 32; o > 0; o >>= 1) ph += __shfl_down(ph, o, 64);
+  if (lane_id() == 0) red[threadIdx.x / 64] = ph;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = red[0] + red[1] + red[2] + red[3];
+    if (t) atomicAdd(&phantom[r], t);
+  }
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_b_scan_tiles(const hipns_run_desc_t* runs,
+                                                               const hipns_run_state_t* states, const int32_t* deg,
+                                                               const int32_t* perm, int64_t S, int64_t T,
+                                                               int64_t* tile_sums) {
+  __shared__ int64_t red[SCAN_THREADS / 64];
+  const int r = blockIdx.y;
+  const int64_t n = batch_M(runs, states, r) + 1;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  int64_t acc = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
+    if (i < n) acc += crossed_term(deg + r * S, perm + r * S, i);
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if (lane_id() == 0) red[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int i = 0; i < SCAN_THREADS / 64; ++i) t += red[i];
+    tile_sums[r * T + blockIdx.x] = t;
+  }
+}
+
+__global__ void k_b_scan_sums(int64_t* tile_sums_all, int64_t T) {
+  __shared__ int64_t buf[1024];
+  __shared__ int64_t carry;
+  int64_t* tile_sums = tile_sums_all + (int64_t)blockIdx.y * T;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < T; c0 += 1024) {
+    int64_t i = c0 + threadIdx.x;
+    int64_t v = i < T ? tile_sums[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int64_t add = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += add;
+      __syncthreads();
+    }
+    int64_t incl = buf[threadIdx.x];
+    if (i < T) tile_sums[i] = carry + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += incl;
+    __syncthreads();
+  }
+}
+
+// num_live_points of sorted sample i (the static form: every node counts)
+__global__ void __launch_bounds__(SCAN_THREADS) k_b_scan_apply(const hipns_run_desc_t* runs,
+                                                               const hipns_run_state_t* states, const int32_t* deg,
+                                                               const int32_t* perm, int64_t S, int64_t T,
+                                                               const int64_t* tile_sums, int64_t* num_live) {
+  __shared__ int64_t tsum[SCAN_THREADS];
+  const int r = blockIdx.y;
+  const int64_t n = batch_M(runs, states, r) + 1;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int64_t local[SCAN_ITEMS];
+  int64_t acc = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = base + k;
+    int64_t v = i < n ? crossed_term(deg + r * S, perm + r * S, i) : 0;
+    acc += v;
+    local[k] = acc;
+  }
+  tsum[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 1; o < SCAN_THREADS; o <<= 1) {
+    int64_t add = threadIdx.x >= o ? tsum[threadIdx.x - o] : 0;
+    __syncthreads();
+    tsum[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const int64_t pre = tile_sums[r * T + blockIdx.x] + tsum[threadIdx.x] - acc;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = base + k;
+    if (i >= n - 1) break;  // the last node is dropped
+    num_live[r * S + i] = 1 + pre + local[k];
+  }
+}
+
+__global__ void __launch_bounds__(EV_THREADS) k_b_evidence_scan(const hipns_run_desc_t* runs,
+                                                                const hipns_run_state_t* states, const double* samp_logL,
+                                                                int64_t S, const int32_t* perm, const int64_t* num_live,
+                                                                TileStatus* status, int64_t Tev, uint32_t* counters,
+                                                                double* fin) {
+  const int r = blockIdx.y;
+  const int64_t M = batch_M(runs, states, r);
+  double* fin8 = fin + 8 * r;
+  if (M == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const double ninf = -__builtin_inf();
+      const double v[8] = {ninf, 0.0, 0.0, ninf, ninf, ninf, ninf, ninf};
+      for (int i = 0; i < 8; ++i) fin8[i] = v[i];
+    }
+    return;
+  }
+  const EvSrcSorted src{samp_logL + runs[r].samp_off, perm + r * S, num_live + r * S};
+  evidence_scan_tile(src, M, M, nullptr, fin8, status + r * Tev, counters + r);
+}
+
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+struct BatchEvLayout {
+  int64_t S, Tscan, Tev;
+  size_t zero_bytes, total;
+  size_t off_status, off_counters, off_kA, off_kB, off_vA, off_vB, off_nl, off_ts;
+};
+
+static BatchEvLayout batch_ev_layout(int R, int64_t max_M) {
+  BatchEvLayout L{};
+  L.S = round_up(max_M + 1, SORT_TILE);
+  L.Tscan = (L.S + SCAN_TILE - 1) / SCAN_TILE;
+  L.Tev = (L.S + EV_TILE - 1) / EV_TILE;
+  size_t o = 0;
+  o += (size_t)R * L.S * sizeof(int32_t);  // deg
+  o = round_up(o, 256);
+  L.off_status = o;
+  o += (size_t)R * L.Tev * sizeof(TileStatus);
+  o = round_up(o, 256);
+  L.off_counters = o;
+  o += (size_t)R * sizeof(uint32_t);
+  o = round_up(o, 256);
+  L.zero_bytes = o;
+  L.off_kA = o;
+  o += (size_t)R * L.S * sizeof(double);
+  L.off_kB = o;
+  o += (size_t)R * L.S * sizeof(double);
+  L.off_vA = o;
+  o += (size_t)R * L.S * sizeof(int32_t);
+  L.off_vB = o;
+  o += (size_t)R * L.S * sizeof(int32_t);
+  o = round_up(o, 256);
+  L.off_nl = o;
+  o += (size_t)R * L.S * sizeof(int64_t);
+  L.off_ts = o;
+  o += (size_t)R * L.Tscan * sizeof(int64_t);
+  L.total = round_up(o, 256);
+  return L;
+}
+
+size_t batch_evidence_ws(int R, int64_t max_M) { return batch_ev_layout(R, max_M).total; }
+
+hipError_t batch_evidence(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int R, int64_t max_M,
+                          const hipns_buffers_t& B, double* fin, int64_t* phantom, void* ws, hipStream_t s) {
+  const BatchEvLayout L = batch_ev_layout(R, max_M);
+  char* base = (char*)ws;
+  int32_t* deg = (int32_t*)base;
+  TileStatus* status = (TileStatus*)(base + L.off_status);
+  uint32_t* counters = (uint32_t*)(base + L.off_counters);
+  double* kA = (double*)(base + L.off_kA);
+  double* kB = (double*)(base + L.off_kB);
+  int32_t* vA = (int32_t*)(base + L.off_vA);
+  int32_t* vB = (int32_t*)(base + L.off_vB);
+  int64_t* nl = (int64_t*)(base + L.off_nl);
+  int64_t* ts = (int64_t*)(base + L.off_ts);
+  const int64_t words = (int64_t)(L.zero_bytes / 4);
+  hipLaunchKernelGGL(k_fill_zero_u32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, (uint32_t*)base, words);
+  hipLaunchKernelGGL(k_fill_zero_u32, dim3((unsigned)((2 * R + 255) / 256)), dim3(256), 0, s, (uint32_t*)phantom,
+                     (int64_t)2 * R);
+  // nodes into kA; sort: tile pass writes (k0, v0), merges ping-pong; the final buffers hold the permutation
+  hipLaunchKernelGGL(k_b_nodes, dim3((unsigned)((L.S + 255) / 256), R), dim3(256), 0, s, runs, states, B.samp_logL,
+                     L.S, kA);
+  const int64_t tiles = L.S / SORT_TILE;
+  hipLaunchKernelGGL(k_b_tile_sort, dim3((unsigned)tiles, R), dim3(SORT_THREADS), 0, s, runs, states, kA, L.S, kB, vB);
+  double* k0 = kB;
+  int32_t* v0 = vB;
+  double* k1 = kA;
+  int32_t* v1 = vA;
+  for (int64_t run = SORT_TILE; run < L.S; run <<= 1) {
+    hipLaunchKernelGGL(k_b_merge_pass, dim3((unsigned)((L.S + 255) / 256), R), dim3(256), 0, s, runs, states, k0, v0,
+                       L.S, run, k1, v1);
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  const int32_t* perm = v0;
+  hipLaunchKernelGGL(k_b_out_degree, dim3((unsigned)((L.S + 255) / 256), R), dim3(256), 0, s, runs, states, B, L.S,
+                     deg, (unsigned long long*)phantom);
+  hipLaunchKernelGGL(k_b_scan_tiles, dim3((unsigned)L.Tscan, R), dim3(SCAN_THREADS), 0, s, runs, states, deg, perm,
+                     L.S, L.Tscan, ts);
+  hipLaunchKernelGGL(k_b_scan_sums, dim3(1, R), dim3(1024), 0, s, ts, L.Tscan);
+  hipLaunchKernelGGL(k_b_scan_apply, dim3((unsigned)L.Tscan, R), dim3(SCAN_THREADS), 0, s, runs, states, deg, perm,
+                     L.S, L.Tscan, ts, nl);
+  hipLaunchKernelGGL(k_b_evidence_scan, dim3((unsigned)L.Tev, R), dim3(EV_THREADS), 0, s, runs, states, B.samp_logL,
+                     L.S, perm, nl, status, L.Tev, counters, fin);
   return hipGetLastError();
 }
 

@@ -66,10 +66,14 @@ struct LinRegLds {
   const float* r0;     // [npad] r0 = y - X w0 (computed in FP64, zero on padding rows)
   const uint16_t* xh;  // [npad][xs] bf16 hi part of X
   const uint16_t* xl;  // [npad][xs] bf16 lo part of X - hi
+  const float* y;      // [npad] targets (zero on padding rows), read by the adjoint statistics only
   int npad, xs;
   int terms;           // 3: BF16x3; 4: adds X hi times the third bf16 part of w - w0
 };
 
+hipError_t launch_linreg_stats(const float* U, int64_t B, double* logL, double* rss, double* yr,
+                               const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int lpc, int dpmax,
+                               size_t lds, hipStream_t s, const LinRegLds& lr);
 hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
                          const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s,
                          const LinRegLds& lr);

@@ -115,9 +115,10 @@ __device__ __forceinline__ double chain_sum_d(double v) {
 // the 16-bit weight split that every row shares): LPC=2 -> v_mfma_f32_32x32x16_bf16 over 32-row tiles, LPC=4 -> v_mfma_f32_16x16x32_bf16
 // over 16-row tiles.  The G wavefronts of the block share the chains and split the rows; each row
 // tile's 16 (4) squares are summed in FP32 and accumulated in FP64.
-template <int LPC, int NJ>
+template <int LPC, int NJ, bool STATS = false>
 __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int D, const LinRegLds& P, uint32_t tfm,
-                                             int g, int G, double log_const) {
+                                             int g, int G, double log_const, double* rss = nullptr,
+                                             double* yr = nullptr) {
   constexpr int CW = HIPNS_WAVE / LPC;
   constexpr int NG = NJ / 8;  // k-groups of 8 owned dims
   const int l = lane_id();
@@ -148,12 +149,12 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
   const int ngroups = (nj + 7) / 8;
   const int tiles = P.npad / CW;
   const int t0 = (int)(((long long)tiles * g) / G), t1 = (int)(((long long)tiles * (g + 1)) / G);
-  double ss = 0.0;
+  double ss = 0.0, sy = 0.0;
   for (int tl = t0; tl < t1; ++tl) {
     const int row = tl * CW + m;
     const uint16_t* xh = P.xh + (int64_t)row * P.xs + 8 * h;
     const uint16_t* xl = P.xl + (int64_t)row * P.xs + 8 * h;
-    float part = 0.0f;
+    float part = 0.0f, ypart = 0.0f;
     if constexpr (LPC == 2) {
       f32x16 acc = {0};
 #pragma unroll
@@ -169,8 +170,10 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
       }
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        float r = P.r0[tl * 32 + 8 * (i >> 2) + 4 * h + (i & 3)] - acc[i];
+        const int ri = tl * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+        float r = P.r0[ri] - acc[i];
         part = fmaf(r, r, part);
+        if (STATS) ypart = fmaf(P.y[ri], r, ypart);
       }
     } else {
       f32x4 acc = {0};
@@ -187,13 +190,21 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        float r = P.r0[tl * 16 + 4 * h + i] - acc[i];
+        const int ri = tl * 16 + 4 * h + i;
+        float r = P.r0[ri] - acc[i];
         part = fmaf(r, r, part);
+        if (STATS) ypart = fmaf(P.y[ri], r, ypart);
       }
     }
     ss += (double)part;
+    if (STATS) sy += (double)ypart;
   }
   ss = chain_sum_d<LPC>(ss);
+  if constexpr (STATS) {
+    // adjoint statistics (single wavefront per chain group): |r|^2 and y^T r
+    *rss = ss;
+    *yr = chain_sum_d<LPC>(sy);
+  }
   if (G > 1) {
     if (h == 0) P.partial[g * CW + m] = ss;
     __syncthreads();

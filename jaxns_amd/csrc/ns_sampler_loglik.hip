@@ -62,6 +62,64 @@ static hipError_t launch_loglik_dp(const float* U, int64_t B, double* out, const
   return hipGetLastError();
 }
 
+// adjoint statistics of the regression likelihood over a batch of points: logL, |r|^2 and y^T r
+// per row of U (one wavefront per chain group; the host derives d logL / d(log sigma, log tau))
+template <int LPC, int NJ>
+__global__ void __launch_bounds__(64) k_linreg_stats(const float* U, int64_t B, double* logL, double* rss, double* yr,
+                                                     const hipns_run_desc_t* runs, hipns_buffers_t Bf, LinRegLds lrd) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  extern __shared__ float lds[];
+  const hipns_run_desc_t run = runs[0];
+  const int dps = Bf.dps;
+  const int D = run.D;
+  const int DPr = ((D + 8 * LPC - 1) / (8 * LPC)) * (8 * LPC);
+  const int nj = DPr / LPC;
+  MvnLds mv;
+  GmmLds gm;
+  LinRegLds lr = lrd;
+  load_params(HIPNS_FAM_LINREG, Bf.params + run.param_off, dps, D, 1, lds, mv, gm, LPC, &lr);
+  const int l = lane_id();
+  const int c = l % CW, h = l / CW;
+  const int64_t row = (int64_t)blockIdx.x * CW + c;
+  float up[NJ];
+  const bool ok = row < B;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    int d = owned_dim<LPC>(h, j);
+    up[j] = (ok && j < nj && d < D) ? U[row * dps + d] : 0.5f;
+  }
+  const uint32_t tfm = transform_mask<LPC, NJ>(lr.tf, h, nj);
+  double ss = 0.0, sy = 0.0;
+  const double L = lik_linreg<LPC, NJ, true>(up, nj, D, lr, tfm, 0, 1, run.log_const, &ss, &sy);
+  if (ok && h == 0) {
+    logL[row] = L;
+    rss[row] = ss;
+    yr[row] = sy;
+  }
+}
+
+hipError_t launch_linreg_stats(const float* U, int64_t B, double* logL, double* rss, double* yr,
+                               const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int lpc, int dpmax,
+                               size_t lds, hipStream_t s, const LinRegLds& lr) {
+  if (dpmax > 64) lpc = 4;
+  const int CW = HIPNS_WAVE / lpc;
+  dim3 grid((unsigned)((B + CW - 1) / CW)), block(HIPNS_WAVE);
+  if (lpc == 2) {
+    if (dpmax <= 32)
+      hipLaunchKernelGGL((k_linreg_stats<2, 16>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
+    else
+      hipLaunchKernelGGL((k_linreg_stats<2, 32>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
+  } else {
+    if (dpmax <= 32)
+      hipLaunchKernelGGL((k_linreg_stats<4, 8>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
+    else if (dpmax <= 64)
+      hipLaunchKernelGGL((k_linreg_stats<4, 16>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
+    else
+      hipLaunchKernelGGL((k_linreg_stats<4, 32>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
                          const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s,
                          const LinRegLds& lr) {

@@ -29,5 +29,8 @@ size_t argsort_ws(int64_t n);
 hipError_t count_crossed_edges(const int32_t* sender, const double* logL, int64_t N, int64_t num_samples,
                                int64_t* samples_indices, int64_t* num_live, void* ws, size_t ws_bytes, hipStream_t s);
 size_t count_crossed_edges_ws(int64_t N);
+size_t batch_evidence_ws(int R, int64_t max_M);
+hipError_t batch_evidence(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int R, int64_t max_M,
+                          const hipns_buffers_t& B, double* fin, int64_t* phantom, void* ws, hipStream_t s);
 
 }  // namespace hipns

@@ -197,6 +197,7 @@ class DeviceBatch:
         self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, lr_ndata, max_steps, lr_xs,
                              lr_terms, lr_off)
         self._graph = None
+        self._ev_ws = None
         self._graph_iters = 0
         self._graph_term = None
 
@@ -295,8 +296,34 @@ class DeviceBatch:
 
     # ------------------------------------------------------------------ results
     def log_Z(self, stream=None) -> np.ndarray:
-        """Per-run (log_Z_mean, log_Z_uncert): every run's device tree count and evidence
-        scan is enqueued back to back on the stream; one host copy at the end."""
+        """Per-run (log_Z_mean, log_Z_uncert) of _to_results (sharded_static.py:706-745): the tree
+        count and evidence scan of every run in one set of launches (hipns_batch_evidence), one
+        host copy at the end."""
+        torch = _torch()
+        dev = self.device
+        if self._ev_ws is None:
+            nbytes = int(lib().hipns_batch_evidence_workspace_size(self.R, self.runs_host))
+            self._ev_ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        fins = torch.empty((self.R, 8), dtype=torch.float64, device=dev)
+        phs = torch.empty(self.R, dtype=torch.int64, device=dev)
+        check(lib().hipns_batch_evidence(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
+                                         C.byref(self.bufs), ptr(fins), ptr(phs), ptr(self._ev_ws),
+                                         self._ev_ws.numel(), self._s(stream)), "hipns_batch_evidence")
+        fin = fins.cpu().numpy()
+        ph = phs.cpu().numpy()
+        sts = self.states()
+        out = []
+        for r in range(self.R):
+            M = int(min(sts[r].num_samples, self.runs_host[r].max_samples))
+            mu, var = _linear_to_log_stats(fin[r, 3], fin[r, 5])
+            pf = ph[r] / max(M, 1)
+            k = pf / (1.0 - pf)
+            out.append((mu, math.sqrt(var) * math.sqrt(1.0 + k)))
+        return np.array(out)
+
+    def log_Z_per_run(self, stream=None) -> np.ndarray:
+        """log_Z through the single-run primitives (hipns_count_crossed_edges + hipns_evidence_scan)
+        for every run in turn; the batched path must agree with it exactly."""
         torch = _torch()
         sts = self.states()
         dev = self.device
@@ -321,8 +348,7 @@ class DeviceBatch:
             nld[:M].copy_(nl[:M])
             check(lib().hipns_evidence_scan(ptr(lL), ptr(nld), M, -1, None, ptr(fins[r]), ptr(ws_e), ws_e.numel(),
                                             s), "hipns_evidence_scan")
-            if self.specs[r].k_phantom > 0:
-                phs[r] = self.samp_phantom[off:off + M].sum()
+            phs[r] = self.samp_phantom[off:off + M].sum()
         fin = fins.cpu().numpy()
         ph = phs.cpu().numpy()
         out = []
@@ -331,7 +357,7 @@ class DeviceBatch:
             pf = ph[r] / Ms[r]
             k = pf / (1.0 - pf)
             out.append((mu, math.sqrt(var) * math.sqrt(1.0 + k)))
-        return np.array(out)
+        return np.array(out), fin, ph
 
     def results_core(self, r: int, stream=None, full: bool = False) -> dict:
         """_to_results (sharded_static.py:652-745, trim=True) for run r, on the device."""

@@ -155,7 +155,7 @@ class RegressionData:
         self._packed = {}
 
     def packed(self, dps: int):
-        """[r0: npad][X hi: npad x xs bf16][X lo: npad x xs bf16] as float32 words, npad, xs."""
+        """[r0: npad][X hi: npad x xs bf16][X lo: npad x xs bf16][y: npad] as float32 words, npad, xs."""
         if dps in self._packed:
             return self._packed[dps]
         npad = -(-self.N // 32) * 32
@@ -166,7 +166,9 @@ class RegressionData:
         Xs[:self.N, :self.D] = self.X / self.col_scale
         hi = _bf16_rne(Xs)
         lo = _bf16_rne(Xs - _bf16_to_f32(hi))
-        blk = np.concatenate([r0, hi.ravel().view(np.float32), lo.ravel().view(np.float32)])
+        yp = np.zeros(npad, np.float32)
+        yp[:self.N] = self.y
+        blk = np.concatenate([r0, hi.ravel().view(np.float32), lo.ravel().view(np.float32), yp])
         self._packed[dps] = (blk, npad, xs)
         return self._packed[dps]
 

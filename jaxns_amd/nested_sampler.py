@@ -41,7 +41,7 @@ class NestedSampler:
                  c: Optional[int] = None, devices=None, difficult_model: bool = False,
                  parameter_estimation: bool = False, shell_fraction: float = 0.5, gradient_guided: bool = False,
                  init_efficiency_threshold: float = 0.1, verbose: bool = False, *, lanes_per_chain: int = 2,
-                 waves_per_block: int = 4, max_steps: int = 512):
+                 waves_per_block: int = 4, max_steps: int = 512, lr_terms: int = 3):
         self.model = model
         D = model.U_ndims
         # public.py:69-107
@@ -85,7 +85,8 @@ class NestedSampler:
         self.shell_fraction = shell_fraction
         self.init_efficiency_threshold = init_efficiency_threshold
         self.verbose = verbose
-        self.launch_kw = dict(lanes_per_chain=lanes_per_chain, waves_per_block=waves_per_block, max_steps=max_steps)
+        self.launch_kw = dict(lanes_per_chain=lanes_per_chain, waves_per_block=waves_per_block, max_steps=max_steps,
+                              lr_terms=lr_terms)
         probe = self._spec(0)
         self.num_live_points = probe.num_live_points
         self._max_samples_rounded = probe.max_samples

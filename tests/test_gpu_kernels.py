@@ -349,6 +349,38 @@ def test_linreg_hyperparameter_batch_logZ():
     assert all(st.termination_reason & 4 for st in b.states())
 
 
+def test_batch_evidence_matches_per_run():
+    """hipns_batch_evidence (all runs in one set of launches) against the single-run primitives
+    (count_crossed_edges + evidence scan per run), on uneven runs with phantom samples (k=2), so
+    the samples are not already sorted: phantom counts exact, evidence statistics to 1e-12."""
+    import torch
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    say("batched evidence vs per run")
+    rng = np.random.default_rng(77)
+    specs = []
+    for r in range(5):
+        D = int(rng.integers(2, 7))
+        N = int(rng.integers(100, 300))
+        model, _, _, _ = gmm_problem(D, seed=500 + r)
+        specs.append(RunSpec(model=model, num_live_points=N, num_slices=4 * D, k_phantom=2, max_samples=N * 200,
+                             seed=r))
+    b = DeviceBatch(specs, lanes_per_chain=4, waves_per_block=2)
+    b.run(dict(dlogZ=math.log(1.0 + 1e-2), max_samples=1e18), graph_iters=4)
+    z1, fin1, ph1 = b.log_Z_per_run()
+    fins = torch.empty((b.R, 8), dtype=torch.float64, device="cuda")
+    phs = torch.empty(b.R, dtype=torch.int64, device="cuda")
+    ws = torch.empty(int(lib().hipns_batch_evidence_workspace_size(b.R, b.runs_host)), dtype=torch.uint8,
+                     device="cuda")
+    check(lib().hipns_batch_evidence(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host, C.byref(b.bufs),
+                                     ptr(fins), ptr(phs), ptr(ws), ws.numel(), _stream()))
+    fin2, ph2 = fins.cpu().numpy(), phs.cpu().numpy()
+    say("phantoms", ph1, ph2)
+    assert np.array_equal(ph1, ph2) and np.all(ph2 > 0)
+    np.testing.assert_allclose(fin2, fin1, rtol=1e-12, atol=1e-12)
+    z2 = b.log_Z()
+    np.testing.assert_allclose(z2, z1, rtol=1e-12, atol=1e-12)
+
+
 def test_uniform_sampler_phase():
     """UniformSampler shells (samplers/uniform_samplers.py:42-85): every new sample satisfies
     the acceptance rule (logL > contour, or the cap of 100 likelihood evaluations)."""

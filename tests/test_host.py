@@ -146,10 +146,11 @@ def test_linreg_device_packing_reproduces_likelihood():
         D, dps = 20, 32
         prm, c = lik.device_params(model, dps)
         blk, npad, xs = lik.data.packed(dps)
-        assert npad == 320 and xs == 32 and blk.size == npad + npad * xs
+        assert npad == 320 and xs == 32 and blk.size == 2 * npad + npad * xs
+        np.testing.assert_array_equal(blk[npad + npad * xs:npad + npad * xs + 300], lik.y.astype(np.float32))
         r0 = blk[:npad].astype(np.float64)
         hi = blk[npad:npad + npad * xs // 2].view(np.uint16).reshape(npad, xs)
-        lo = blk[npad + npad * xs // 2:].view(np.uint16).reshape(npad, xs)
+        lo = blk[npad + npad * xs // 2:npad + npad * xs].view(np.uint16).reshape(npad, xs)
         Xh, Xl = _bf16_to_f32(hi).astype(np.float64), _bf16_to_f32(lo).astype(np.float64)
         p, s_, w0 = prm[dps:2 * dps], prm[2 * dps:3 * dps], prm[3 * dps:4 * dps]
         assert np.all(prm[:D] == 1) and prm[4 * dps] == np.float32(0.5 / 0.3 ** 2)
