@@ -61,7 +61,7 @@ struct LinRegLds {
   float* p;            // [dps] w = p + s (.) q(u)
   float* s;            // [dps]
   float* w0;           // [dps] centre of the residual GEMM
-  float inv2s2;        // 1 / (2 sigma^2)
+  double inv2s2;       // 1 / (2 sigma^2), held as FP64 so that logL is smooth in sigma
   double* partial;     // [G][CW] per-wavefront partial sums of squares
   const float* r0;     // [npad] r0 = y - X w0 (computed in FP64, zero on padding rows)
   const uint16_t* xh;  // [npad][xs] bf16 hi part of X

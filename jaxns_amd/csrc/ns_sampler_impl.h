@@ -212,7 +212,7 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
     for (int gg = 0; gg < G; ++gg) ss += P.partial[gg * CW + m];
     __syncthreads();
   }
-  double L = log_const - ss * (double)P.inv2s2;
+  double L = log_const - ss * P.inv2s2;
   return __builtin_isnan(L) ? -__builtin_inf() : L;
 }
 
@@ -347,7 +347,9 @@ __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, 
     lr->partial = (double*)(lds + 4 * dps + 4);
     for (int i = tid; i < 4 * dps + 4; i += nt) lds[i] = prm[i];
     __syncthreads();
-    lr->inv2s2 = lds[4 * dps];
+    // 1/(2 sigma^2) as the two 32-bit words of an FP64 value (little-endian)
+    lr->inv2s2 = __longlong_as_double(((long long)__float_as_uint(lds[4 * dps + 1]) << 32) |
+                                      (long long)__float_as_uint(lds[4 * dps]));
     gm.tf = lr->tf;
     return;
   }

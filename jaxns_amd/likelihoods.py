@@ -177,7 +177,7 @@ class LinearRegressionLikelihood(LikelihoodFamily):
     """-N/2 log(2 pi sigma^2) - |y - X w|^2 / (2 sigma^2) over the weights w (the prior's output).
 
     Runs of one batch share the RegressionData object (pass data=...), and may differ in sigma and
-    in their priors.  Device layout per run: [q-kind: dps][p: dps][s: dps][w0: dps][1/(2 sig^2), 0, 0, 0]
+    in their priors.  Device layout per run: [q-kind: dps][p: dps][s: dps][w0: dps][1/(2 sig^2) as FP64, 0, 0]
     in column-scaled coordinates, plus the shared block of RegressionData.packed.
     """
 
@@ -211,7 +211,7 @@ class LinearRegressionLikelihood(LikelihoodFamily):
         prm[dps:dps + D] = loc * c
         prm[2 * dps:2 * dps + D] = np.diag(S) * c
         prm[3 * dps:3 * dps + D] = self.data.w0 * c
-        prm[4 * dps] = 0.5 / self.sigma ** 2
+        prm[4 * dps:4 * dps + 2] = np.array([0.5 / self.sigma ** 2], np.float64).view(np.float32)
         return prm, float(self.log_const)
 
     def log_evidence_normal_prior(self, tau) -> float:

@@ -153,7 +153,8 @@ def test_linreg_device_packing_reproduces_likelihood():
         lo = blk[npad + npad * xs // 2:npad + npad * xs].view(np.uint16).reshape(npad, xs)
         Xh, Xl = _bf16_to_f32(hi).astype(np.float64), _bf16_to_f32(lo).astype(np.float64)
         p, s_, w0 = prm[dps:2 * dps], prm[2 * dps:3 * dps], prm[3 * dps:4 * dps]
-        assert np.all(prm[:D] == 1) and prm[4 * dps] == np.float32(0.5 / 0.3 ** 2)
+        inv2s2 = float(prm[4 * dps:4 * dps + 2].view(np.float64)[0])
+        assert np.all(prm[:D] == 1) and inv2s2 == 0.5 / 0.3 ** 2
         rng = np.random.default_rng(1)
         # prior draws and posterior draws
         A = np.eye(D) / 1.5 ** 2 + lik.X.T @ lik.X / 0.3 ** 2
@@ -167,7 +168,7 @@ def test_linreg_device_packing_reproduces_likelihood():
             dh = _bf16_to_f32(_bf16_rne(dw)).astype(np.float64)
             dl = _bf16_to_f32(_bf16_rne((dw - dh).astype(np.float32))).astype(np.float64)
             r = r0 - (Xh @ dh + Xh @ dl + Xl @ dh)
-            got = c - np.sum(r * r) * float(prm[4 * dps])
+            got = c - np.sum(r * r) * inv2s2
             want = norm.logpdf(lik.y, lik.X @ w, 0.3).sum()
             tol = (3e-5 if scaled else 1e-5) * abs(want) if i < 20 else 1e-6 * abs(want) + 1e-3
             assert abs(got - want) <= tol, (scaled, i, got, want)
