@@ -128,13 +128,15 @@ typedef struct hipns_buffers {
 typedef struct hipns_launch {
   int32_t family;
   int32_t lanes_per_chain;
-  int32_t waves_per_block;  /* MVN/GMM: chain groups per workgroup; LinReg: wavefronts sharing one group's rows */
+  int32_t waves_per_block;  /* chain groups (one per wavefront) per workgroup; LinReg multiplies each row
+                             * tile against all of them */
   int32_t gmm_K;
   int32_t lr_ndata;         /* LinReg: data rows padded to a multiple of 32 */
   int32_t max_steps;        /* step cap per slice (safety bound on the shrink loop) */
-  int32_t lr_xstride;       /* LinReg: bf16 elements per X row (a multiple of 32) */
+  int32_t lr_panels;        /* LinReg: 16-column panels of X (even, 16 * lr_panels >= D rounded up to 32) */
   int32_t lr_terms;         /* LinReg: 3 (0) = BF16x3 hi.hi + hi.lo + lo.hi; 4 adds X hi . (third 8 bits of w) */
-  int64_t lr_data_off;      /* LinReg: float offset in params of [r0 | X hi | X lo | y] shared by all runs */
+  int64_t lr_data_off;      /* LinReg: float offset in params of [r0 | X hi | X lo | y] shared by all runs;
+                             * X parts are [lr_panels][lr_ndata][16] bf16 */
 } hipns_launch_t;
 
 /* ---------------------------------------------------------------------

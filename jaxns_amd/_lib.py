@@ -65,7 +65,7 @@ class Buffers(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("family", _i32), ("lanes_per_chain", _i32), ("waves_per_block", _i32), ("gmm_K", _i32),
-                ("lr_ndata", _i32), ("max_steps", _i32), ("lr_xstride", _i32), ("lr_terms", _i32),
+                ("lr_ndata", _i32), ("max_steps", _i32), ("lr_panels", _i32), ("lr_terms", _i32),
                 ("lr_data_off", _i64)]
 
 

@@ -73,7 +73,8 @@ int check_launch(const hipns_launch_t* L) {
     return fail(HIPNS_ENOTSUP, "family %d not supported by the fused sampler", L->family);
   if (L->family == HIPNS_FAM_LINREG) {
     if (L->lr_ndata < 32 || L->lr_ndata % 32) return fail(HIPNS_EINVAL, "lr_ndata must be a positive multiple of 32");
-    if (L->lr_xstride < 32 || L->lr_xstride % 32) return fail(HIPNS_EINVAL, "lr_xstride must be a multiple of 32");
+    if (L->lr_panels < 2 || L->lr_panels % 2 || L->lr_panels > HIPNS_MAX_DIM / 16)
+      return fail(HIPNS_EINVAL, "lr_panels must be even, 2..%d", HIPNS_MAX_DIM / 16);
     if (L->lr_data_off < 0 || L->lr_data_off % 4) return fail(HIPNS_EINVAL, "lr_data_off must be a multiple of 4");
     if (L->lr_terms != 0 && L->lr_terms != 3 && L->lr_terms != 4) return fail(HIPNS_EINVAL, "lr_terms must be 3 or 4");
   }
@@ -86,20 +87,29 @@ int check_launch(const hipns_launch_t* L) {
   return HIPNS_OK;
 }
 
-// device view of the shared regression rows: params + lr_data_off = [r0: npad][X hi: npad x xs bf16]
-// [X lo: npad x xs bf16][y: npad]
+// device view of the shared regression rows: params + lr_data_off = [r0: npad]
+// [X hi: panels x npad x 16 bf16][X lo: same][y: npad]
 hipns::LinRegLds linreg_view(const hipns_buffers_t* bufs, const hipns_launch_t* L) {
   hipns::LinRegLds lr{};
   const float* base = bufs->params + L->lr_data_off;
-  const int64_t npad = L->lr_ndata, xs = L->lr_xstride;
+  const int64_t npad = L->lr_ndata, xe = (int64_t)L->lr_panels * npad * 16;
   lr.r0 = base;
   lr.xh = (const uint16_t*)(base + npad);
-  lr.xl = lr.xh + npad * xs;
-  lr.y = base + npad + npad * xs;
+  lr.xl = lr.xh + xe;
+  lr.y = base + npad + xe;
   lr.npad = L->lr_ndata;
-  lr.xs = L->lr_xstride;
+  lr.panels = L->lr_panels;
   lr.terms = L->lr_terms == 4 ? 4 : 3;
   return lr;
+}
+
+// every column a kernel instance reads (NJ * LPC of the instance chosen for maxD) must lie inside the
+// panels
+int check_panels(const hipns_launch_t* L, int maxD, int lpc) {
+  const int cols = hipns::sampler_nj(maxD, lpc) * lpc;
+  if (16 * L->lr_panels < cols)
+    return fail(HIPNS_EINVAL, "lr_panels=%d cannot hold the %d columns read for D=%d", L->lr_panels, cols, maxD);
+  return HIPNS_OK;
 }
 
 int max_lds_bytes() {
@@ -114,13 +124,14 @@ int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A
   int lpc = (dps > 64) ? 4 : L->lanes_per_chain;
   hipns::SampleArgs A = A0;
   if (L->family == HIPNS_FAM_LINREG) {
-    if (L->lr_xstride < dps) return fail(HIPNS_EINVAL, "lr_xstride %d below dps %d", L->lr_xstride, dps);
+    int rc = check_panels(L, maxD, lpc);
+    if (rc) return rc;
     const hipns::LinRegLds lr = linreg_view(&A.B, L);
     A.lr_r0 = lr.r0;
     A.lr_xh = lr.xh;
     A.lr_xl = lr.xl;
     A.lr_npad = lr.npad;
-    A.lr_xs = lr.xs;
+    A.lr_panels = lr.panels;
     A.lr_terms = lr.terms;
   }
   A.alpha_off = (int32_t)hipns::sampler_lds_floats(L->family, dps, maxD, L->gmm_K, lpc, L->waves_per_block);
@@ -131,10 +142,10 @@ int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A
   L2.lanes_per_chain = lpc;
   hipError_t e;
   switch (mode) {
-    case hipns::MODE_SLICE: e = hipns::launch_sampler_slice(A, R, max_chains, L2, dps, lds, s); break;
-    case hipns::MODE_UNIF: e = hipns::launch_sampler_unif(A, R, max_chains, L2, dps, lds, s); break;
-    case hipns::MODE_INIT: e = hipns::launch_sampler_init(A, R, max_chains, L2, dps, lds, s); break;
-    default: e = hipns::launch_sampler_inj(A, R, max_chains, L2, dps, lds, s); break;
+    case hipns::MODE_SLICE: e = hipns::launch_sampler_slice(A, R, max_chains, L2, maxD, lds, s); break;
+    case hipns::MODE_UNIF: e = hipns::launch_sampler_unif(A, R, max_chains, L2, maxD, lds, s); break;
+    case hipns::MODE_INIT: e = hipns::launch_sampler_init(A, R, max_chains, L2, maxD, lds, s); break;
+    default: e = hipns::launch_sampler_inj(A, R, max_chains, L2, maxD, lds, s); break;
   }
   if (e != hipSuccess) return hip_fail(e, "sampler launch");
   return HIPNS_OK;
@@ -447,10 +458,10 @@ int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const 
   L2.lanes_per_chain = lpc;
   hipns::LinRegLds lr{};
   if (launch->family == HIPNS_FAM_LINREG) {
-    if (launch->lr_xstride < bufs->dps) return fail(HIPNS_EINVAL, "lr_xstride below dps");
+    if ((rc = check_panels(launch, D, lpc))) return rc;
     lr = linreg_view(bufs, launch);
   }
-  hipError_t e = hipns::launch_loglik(U_dev, B, logL_dev, runs_dev, *bufs, L2, bufs->dps, lds, as_stream(stream), lr);
+  hipError_t e = hipns::launch_loglik(U_dev, B, logL_dev, runs_dev, *bufs, L2, D, lds, as_stream(stream), lr);
   if (e != hipSuccess) return hip_fail(e, "loglik launch");
   return HIPNS_OK;
 }
@@ -468,10 +479,10 @@ int hipns_linreg_stats(const float* U_dev, int64_t B, double* logL_dev, double* 
   if (!U_dev || !logL_dev || !rss_dev || !yr_dev) return fail(HIPNS_EINVAL, "null pointer");
   const int D = runs_host[0].D;
   if (D < 1 || D > bufs->dps) return fail(HIPNS_EINVAL, "D outside [1, dps]");
-  if (launch->lr_xstride < bufs->dps) return fail(HIPNS_EINVAL, "lr_xstride below dps");
   const int lpc = bufs->dps > 64 ? 4 : launch->lanes_per_chain;
+  if ((rc = check_panels(launch, D, lpc))) return rc;
   const size_t lds = hipns::sampler_lds_floats(HIPNS_FAM_LINREG, bufs->dps, D, 1, lpc, 1) * sizeof(float);
-  hipError_t e = hipns::launch_linreg_stats(U_dev, B, logL_dev, rss_dev, yr_dev, runs_dev, *bufs, lpc, bufs->dps, lds,
+  hipError_t e = hipns::launch_linreg_stats(U_dev, B, logL_dev, rss_dev, yr_dev, runs_dev, *bufs, lpc, D, lds,
                                             as_stream(stream), linreg_view(bufs, launch));
   if (e != hipSuccess) return hip_fail(e, "linreg stats launch");
   return HIPNS_OK;

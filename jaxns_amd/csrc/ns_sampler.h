@@ -26,7 +26,7 @@ struct SampleArgs {
   const uint16_t* lr_xh;
   const uint16_t* lr_xl;
   int32_t lr_npad;
-  int32_t lr_xs;
+  int32_t lr_panels;
   int32_t lr_terms;
   int32_t pad1;
   // injected-draw mode
@@ -43,9 +43,39 @@ struct SampleArgs {
   double* ph_logL;
 };
 
+// regression-family view of the model: per-run vectors and exchange buffers in LDS, the shared data
+// rows in HBM.  X is stored as `panels` column panels of 16: element (row, col) of the hi part is at
+// xh[((col / 16) * npad + row) * 16 + col % 16].
+struct LinRegLds {
+  float* tf;           // [dps] 1 where the prior of the weight is Normal (q = ndtri(u))
+  float* p;            // [dps] w = p + s (.) q(u)
+  float* s;            // [dps]
+  float* w0;           // [dps] centre of the residual GEMM
+  double inv2s2;       // 1 / (2 sigma^2), held as FP64 so that logL is smooth in sigma
+  void* bops;          // [G][NG][3][64] B operands (8 bf16 each) published by every wavefront
+  double* partial;     // [2][G][G][CW] per-wavefront partial sums (|r|^2, then y^T r)
+  const float* r0;     // [npad] r0 = y - X w0 (computed in FP64, zero on padding rows)
+  const uint16_t* xh;  // [panels][npad][16] bf16 hi part of the column-scaled X
+  const uint16_t* xl;  // [panels][npad][16] bf16 lo part (X - hi)
+  const float* y;      // [npad] targets (zero on padding rows), read by the adjoint statistics only
+  int npad, panels;
+  int terms;           // 3: BF16x3; 4: adds X hi times the third bf16 part of w - w0
+};
+
+// register slots per lane of the dispatched kernel instance for runs of at most dmax dimensions
+// (the launchers select the same instance)
+__host__ __device__ inline int sampler_nj(int dmax, int lpc) {
+  if (lpc == 2 && dmax <= 64) return dmax <= 16 ? 8 : (dmax <= 32 ? 16 : 32);
+  return dmax <= 32 ? 8 : (dmax <= 64 ? 16 : 32);
+}
+
 // floats of dynamic LDS one sampler block needs (layout of load_params in ns_sampler_impl.h)
 inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc, int G = 1) {
-  if (fam == HIPNS_FAM_LINREG) return (size_t)4 * dps + 4 + (size_t)2 * G * (64 / lpc);
+  if (fam == HIPNS_FAM_LINREG) {
+    const size_t NG = (size_t)sampler_nj(D, lpc) / 8;
+    const size_t CW = (size_t)(64 / lpc);
+    return (size_t)4 * dps + 4 + (size_t)G * NG * 3 * 64 * 4 + (size_t)2 * G * G * CW * 2;
+  }
   if (fam == HIPNS_FAM_MVN) {
     int CW = 64 / lpc;
     int MB = (D + CW - 1) / CW;
@@ -55,38 +85,22 @@ inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc, int G 
   return (size_t)(3 + K) * dps + 2 * K;
 }
 
-// regression-family view of the model: per-run vectors in LDS, the shared data rows in HBM
-struct LinRegLds {
-  float* tf;           // [dps] 1 where the prior of the weight is Normal (q = ndtri(u))
-  float* p;            // [dps] w = p + s (.) q(u)
-  float* s;            // [dps]
-  float* w0;           // [dps] centre of the residual GEMM
-  double inv2s2;       // 1 / (2 sigma^2), held as FP64 so that logL is smooth in sigma
-  double* partial;     // [G][CW] per-wavefront partial sums of squares
-  const float* r0;     // [npad] r0 = y - X w0 (computed in FP64, zero on padding rows)
-  const uint16_t* xh;  // [npad][xs] bf16 hi part of X
-  const uint16_t* xl;  // [npad][xs] bf16 lo part of X - hi
-  const float* y;      // [npad] targets (zero on padding rows), read by the adjoint statistics only
-  int npad, xs;
-  int terms;           // 3: BF16x3; 4: adds X hi times the third bf16 part of w - w0
-};
-
 hipError_t launch_linreg_stats(const float* U, int64_t B, double* logL, double* rss, double* yr,
-                               const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int lpc, int dpmax,
+                               const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int lpc, int dmax,
                                size_t lds, hipStream_t s, const LinRegLds& lr);
 hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
-                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s,
+                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dmax, size_t lds, hipStream_t s,
                          const LinRegLds& lr);
 
 }  // namespace hipns
 
 namespace hipns {
-hipError_t launch_sampler_slice(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dpmax,
+hipError_t launch_sampler_slice(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
                                 size_t lds_bytes, hipStream_t s);
-hipError_t launch_sampler_unif(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dpmax,
+hipError_t launch_sampler_unif(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
                                size_t lds_bytes, hipStream_t s);
-hipError_t launch_sampler_init(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dpmax,
+hipError_t launch_sampler_init(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
                                size_t lds_bytes, hipStream_t s);
-hipError_t launch_sampler_inj(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dpmax,
+hipError_t launch_sampler_inj(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
                               size_t lds_bytes, hipStream_t s);
 }  // namespace hipns

@@ -110,11 +110,25 @@ __device__ __forceinline__ double chain_sum_d(double v) {
 }
 
 // Gaussian linear regression: r = r0 - X (w - w0) with w = p + s (.) q(u); logL = c - |r|^2 / (2 sigma^2).
-// The GEMM (rows x dims) x (dims x chains) runs as BF16x3 MFMAs (hi.hi + hi.lo + lo.hi, FP32
-// accumulate; terms == 4 adds hi times the third bf16 part of w - w0, which removes the error of
-// the 16-bit weight split that every row shares): LPC=2 -> v_mfma_f32_32x32x16_bf16 over 32-row tiles, LPC=4 -> v_mfma_f32_16x16x32_bf16
-// over 16-row tiles.  The G wavefronts of the block share the chains and split the rows; each row
-// tile's 16 (4) squares are summed in FP32 and accumulated in FP64.
+// The MFMA chain of a tile starts from C = r0 with the weight operands negated, so it ends on r.
+// The G wavefronts of a workgroup each own one group of CW chains.  Every step they publish the
+// B operands of their chains (w - w0 split into bf16 parts) in LDS, then each wavefront takes 1/G of
+// the data-row tiles and multiplies every row tile it loads against all G groups' operands, so X is
+// read once per step per workgroup instead of once per chain group.  Each product is BF16x3
+// (hi.hi + hi.lo + lo.hi, FP32 accumulate; terms == 4 adds hi times the third bf16 part of w - w0,
+// which removes the error of the 16-bit weight split that every row shares): LPC=2 uses
+// v_mfma_f32_32x32x16_bf16 over 32-row tiles, LPC=4 v_mfma_f32_16x16x32_bf16 over 16-row tiles.
+// X is stored in 16-column panels ([panel][row][16] bf16) so both layouts read dense rows.  Squares
+// of a tile are summed in FP32 and accumulated in FP64; the G partial sums of a group meet in LDS
+// in a fixed order.  All G wavefronts must call this together (two workgroup barriers).
+constexpr int LR_MAX_G = 4;
+
+template <int LPC>
+__device__ __forceinline__ const uint16_t* lr_panel_ptr(const uint16_t* x, int npad, int gg, int row, int h) {
+  if constexpr (LPC == 2) return x + ((int64_t)gg * npad + row) * 16 + 8 * h;
+  return x + ((int64_t)(2 * gg + (h >> 1)) * npad + row) * 16 + 8 * (h & 1);
+}
+
 template <int LPC, int NJ, bool STATS = false>
 __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int D, const LinRegLds& P, uint32_t tfm,
                                              int g, int G, double log_const, double* rss = nullptr,
@@ -124,9 +138,13 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
   const int l = lane_id();
   const int m = l % CW;
   const int h = l / CW;
-  bf16x8_t bh[NG], bl[NG], bt[NG];
+  const bool t4 = P.terms == 4;
+  (void)nj;
+  // publish this wavefront's B operands: [G][NG][3][64] vectors of 8 bf16
+  bf16x8_t* Bs = (bf16x8_t*)P.bops;
 #pragma unroll
   for (int gg = 0; gg < NG; ++gg) {
+    bf16x8_t bh, bl, bt;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int j = gg * 8 + i;
@@ -137,82 +155,128 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
         float q = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
         dw = fmaf(P.s[d], q, P.p[d]) - P.w0[d];
       }
+      // published negated, so that the MFMA chain started from C = r0 yields r = r0 - X (w - w0)
       const __bf16 hi = (__bf16)dw;
       const float rem = dw - (float)hi;
       const __bf16 lo = (__bf16)rem;
-      bh[gg][i] = hi;
-      bl[gg][i] = lo;
-      bt[gg][i] = (__bf16)(rem - (float)lo);
+      bh[i] = -hi;
+      bl[i] = -lo;
+      bt[i] = -(__bf16)(rem - (float)lo);
     }
+    Bs[((g * NG + gg) * 3 + 0) * HIPNS_WAVE + l] = bh;
+    Bs[((g * NG + gg) * 3 + 1) * HIPNS_WAVE + l] = bl;
+    Bs[((g * NG + gg) * 3 + 2) * HIPNS_WAVE + l] = bt;
   }
-  const bool t4 = P.terms == 4;
-  const int ngroups = (nj + 7) / 8;
+  __syncthreads();
   const int tiles = P.npad / CW;
   const int t0 = (int)(((long long)tiles * g) / G), t1 = (int)(((long long)tiles * (g + 1)) / G);
-  double ss = 0.0, sy = 0.0;
-  for (int tl = t0; tl < t1; ++tl) {
+  double ss[LR_MAX_G], sy[LR_MAX_G];
+#pragma unroll
+  for (int q = 0; q < LR_MAX_G; ++q) {
+    ss[q] = 0.0;
+    sy[q] = 0.0;
+  }
+  constexpr int NR = (LPC == 2) ? 16 : 4;  // accumulator rows per lane
+  // rows of tile tl held by this lane (the MFMA output layout)
+  auto tile_row = [&](int tl, int i) { return (LPC == 2) ? tl * 32 + 8 * (i >> 2) + 4 * h + (i & 3) : tl * 16 + 4 * h + i; };
+  // software pipeline: the X panels and r0 of the next tile are in flight while this one multiplies.
+  // All NG k-groups are read: the panels cover NJ * LPC columns, the unused ones are zero.
+  bf16x8_t ah[NG], al[NG], nh[NG], nl[NG];
+  float r0v[NR], nr0[NR];
+  auto load_tile = [&](int tl, bf16x8_t (&xh)[NG], bf16x8_t (&xl)[NG], float (&rv)[NR]) {
     const int row = tl * CW + m;
-    const uint16_t* xh = P.xh + (int64_t)row * P.xs + 8 * h;
-    const uint16_t* xl = P.xl + (int64_t)row * P.xs + 8 * h;
-    float part = 0.0f, ypart = 0.0f;
-    if constexpr (LPC == 2) {
-      f32x16 acc = {0};
 #pragma unroll
-      for (int gg = 0; gg < NG; ++gg) {
-        if (gg < ngroups) {
-          bf16x8_t ah = __builtin_bit_cast(bf16x8_t, *(const uint4*)(xh + 16 * gg));
-          bf16x8_t al = __builtin_bit_cast(bf16x8_t, *(const uint4*)(xl + 16 * gg));
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[gg], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[gg], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[gg], acc, 0, 0, 0);
-          if (t4) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bt[gg], acc, 0, 0, 0);
+    for (int gg = 0; gg < NG; ++gg) {
+      xh[gg] = __builtin_bit_cast(bf16x8_t, *(const uint4*)lr_panel_ptr<LPC>(P.xh, P.npad, gg, row, h));
+      xl[gg] = __builtin_bit_cast(bf16x8_t, *(const uint4*)lr_panel_ptr<LPC>(P.xl, P.npad, gg, row, h));
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) rv[i] = P.r0[tile_row(tl, i)];
+  };
+  if (t0 < t1) load_tile(t0, ah, al, r0v);
+  for (int tl = t0; tl < t1; ++tl) {
+    if (tl + 1 < t1) load_tile(tl + 1, nh, nl, nr0);
+    float yv[NR];
+    if (STATS) {
+#pragma unroll
+      for (int i = 0; i < NR; ++i) yv[i] = P.y[tile_row(tl, i)];
+    }
+#pragma unroll
+    for (int q = 0; q < LR_MAX_G; ++q) {
+      if (q < G) {
+        float part = 0.0f, ypart = 0.0f;
+        if constexpr (LPC == 2) {
+          f32x16 acc;
+#pragma unroll
+          for (int i = 0; i < 16; ++i) acc[i] = r0v[i];
+#pragma unroll
+          for (int gg = 0; gg < NG; ++gg) {
+            const bf16x8_t* B = Bs + ((q * NG + gg) * 3) * HIPNS_WAVE + l;
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[gg], B[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[gg], B[HIPNS_WAVE], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[gg], B[0], acc, 0, 0, 0);
+            if (t4) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[gg], B[2 * HIPNS_WAVE], acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const float r = acc[i];
+            part = fmaf(r, r, part);
+            if (STATS) ypart = fmaf(yv[i], r, ypart);
+          }
+        } else {
+          f32x4 acc;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc[i] = r0v[i];
+#pragma unroll
+          for (int gg = 0; gg < NG; ++gg) {
+            const bf16x8_t* B = Bs + ((q * NG + gg) * 3) * HIPNS_WAVE + l;
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[gg], B[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[gg], B[HIPNS_WAVE], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[gg], B[0], acc, 0, 0, 0);
+            if (t4) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[gg], B[2 * HIPNS_WAVE], acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float r = acc[i];
+            part = fmaf(r, r, part);
+            if (STATS) ypart = fmaf(yv[i], r, ypart);
+          }
         }
-      }
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int ri = tl * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-        float r = P.r0[ri] - acc[i];
-        part = fmaf(r, r, part);
-        if (STATS) ypart = fmaf(P.y[ri], r, ypart);
-      }
-    } else {
-      f32x4 acc = {0};
-#pragma unroll
-      for (int gg = 0; gg < NG; ++gg) {
-        if (gg < ngroups) {
-          bf16x8_t ah = __builtin_bit_cast(bf16x8_t, *(const uint4*)(xh + 32 * gg));
-          bf16x8_t al = __builtin_bit_cast(bf16x8_t, *(const uint4*)(xl + 32 * gg));
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[gg], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[gg], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[gg], acc, 0, 0, 0);
-          if (t4) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bt[gg], acc, 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int ri = tl * 16 + 4 * h + i;
-        float r = P.r0[ri] - acc[i];
-        part = fmaf(r, r, part);
-        if (STATS) ypart = fmaf(P.y[ri], r, ypart);
+        ss[q] += (double)part;
+        if (STATS) sy[q] += (double)ypart;
       }
     }
-    ss += (double)part;
-    if (STATS) sy += (double)ypart;
+#pragma unroll
+    for (int gg = 0; gg < NG; ++gg) {
+      ah[gg] = nh[gg];
+      al[gg] = nl[gg];
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) r0v[i] = nr0[i];
   }
-  ss = chain_sum_d<LPC>(ss);
+  // partial sums of every group from this wavefront's rows -> LDS [group][wavefront][chain]
+#pragma unroll
+  for (int q = 0; q < LR_MAX_G; ++q) {
+    if (q < G) {
+      const double v = chain_sum_d<LPC>(ss[q]);
+      const double vy = STATS ? chain_sum_d<LPC>(sy[q]) : 0.0;
+      if (h == 0) {
+        P.partial[(q * G + g) * CW + m] = v;
+        if (STATS) P.partial[(G * G + q * G + g) * CW + m] = vy;
+      }
+    }
+  }
+  __syncthreads();
+  double tot = 0.0, toty = 0.0;
+  for (int k = 0; k < G; ++k) {
+    tot += P.partial[(g * G + k) * CW + m];
+    if (STATS) toty += P.partial[(G * G + g * G + k) * CW + m];
+  }
   if constexpr (STATS) {
-    // adjoint statistics (single wavefront per chain group): |r|^2 and y^T r
-    *rss = ss;
-    *yr = chain_sum_d<LPC>(sy);
+    *rss = tot;
+    *yr = toty;
   }
-  if (G > 1) {
-    if (h == 0) P.partial[g * CW + m] = ss;
-    __syncthreads();
-    ss = 0.0;
-    for (int gg = 0; gg < G; ++gg) ss += P.partial[gg * CW + m];
-    __syncthreads();
-  }
-  double L = log_const - ss * P.inv2s2;
+  double L = log_const - tot * P.inv2s2;
   return __builtin_isnan(L) ? -__builtin_inf() : L;
 }
 
@@ -336,7 +400,7 @@ __device__ __forceinline__ uint32_t transform_mask(const float* tf, int h, int n
 
 // dynamic LDS layout of one block (one run per block)
 __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, int D, int K, float* lds, MvnLds& mv,
-                                            GmmLds& gm, int LPCv, LinRegLds* lr = nullptr) {
+                                            GmmLds& gm, int LPCv, LinRegLds* lr = nullptr, int G = 1, int NG = 1) {
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
   if (fam == HIPNS_FAM_LINREG) {
@@ -344,7 +408,8 @@ __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, 
     lr->p = lds + dps;
     lr->s = lds + 2 * dps;
     lr->w0 = lds + 3 * dps;
-    lr->partial = (double*)(lds + 4 * dps + 4);
+    lr->bops = (void*)(lds + 4 * dps + 4);
+    lr->partial = (double*)(lds + 4 * dps + 4 + G * NG * 3 * 64 * 4);
     for (int i = tid; i < 4 * dps + 4; i += nt) lds[i] = prm[i];
     __syncthreads();
     // 1/(2 sigma^2) as the two 32-bit words of an FP64 value (little-endian)
@@ -388,8 +453,9 @@ __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, 
 template <int LPC, int NJ, int FAM, int MODE>
 __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   constexpr int CW = HIPNS_WAVE / LPC;
-  // the regression family splits the data rows of one chain group over the block's wavefronts
-  constexpr bool SPLITK = (FAM == HIPNS_FAM_LINREG);
+  // the regression family multiplies the data rows against every chain group of the workgroup
+  // cooperatively, so its wavefronts step together (workgroup-uniform loop)
+  constexpr bool COOP = (FAM == HIPNS_FAM_LINREG);
   extern __shared__ float lds[];
   const int r = blockIdx.y;
   const hipns_run_desc_t run = A.runs[r];
@@ -397,15 +463,13 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   if (MODE != MODE_INJ && st->done) return;  // uniform across the block
   const int wpb = blockDim.x / HIPNS_WAVE;
   const int wave = threadIdx.x / HIPNS_WAVE;
-  const int G = SPLITK ? wpb : 1;
-  const int gw = SPLITK ? wave : 0;
-  const bool writer = gw == 0;
+  const int G = COOP ? wpb : 1;
   const int l = lane_id();
   const int c = l % CW;
   const int h = l / CW;
   const int nchains = (MODE == MODE_INIT) ? run.N : ((MODE == MODE_INJ) ? A.inj_C : run.shell);
-  const int chain0 = SPLITK ? blockIdx.x * CW : (blockIdx.x * wpb + wave) * CW;
-  if ((SPLITK ? blockIdx.x * CW : blockIdx.x * wpb * CW) >= nchains) return;  // whole block idle
+  const int chain0 = (blockIdx.x * wpb + wave) * CW;
+  if (blockIdx.x * wpb * CW >= nchains) return;  // whole block idle
   const int dps = A.B.dps;
   const int D = run.D;
   const int DPr = ((D + 8 * LPC - 1) / (8 * LPC)) * (8 * LPC);
@@ -414,13 +478,14 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   MvnLds mv;
   GmmLds gm;
   LinRegLds lr;
-  load_params(FAM, A.B.params + run.param_off, dps, D, A.gmm_K, lds, mv, gm, LPC, &lr);
-  if (SPLITK) {
+  load_params(FAM, A.B.params + run.param_off, dps, D, A.gmm_K, lds, mv, gm, LPC, &lr, G, NJ / 8);
+  if (COOP) {
     lr.r0 = A.lr_r0;
     lr.xh = A.lr_xh;
     lr.xl = A.lr_xl;
+    lr.y = nullptr;
     lr.npad = A.lr_npad;
-    lr.xs = A.lr_xs;
+    lr.panels = A.lr_panels;
     lr.terms = A.lr_terms;
   }
   const uint32_t tfm = transform_mask<LPC, NJ>(FAM == HIPNS_FAM_MVN ? mv.tf : gm.tf, h, nj);
@@ -559,7 +624,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   // Every live chain evaluates exactly once per step, so the step index w equals each live
   // chain's evaluation count: the pick uniform of a chain's w-th evaluation is word (w mod 4)
   // of its Philox block w/4, drawn in the same step by the whole wavefront.
-  for (int w = 0; SPLITK ? __syncthreads_or(alive) : __any(alive); ++w) {
+  for (int w = 0; COOP ? __syncthreads_or(alive) : __any(alive); ++w) {
     if (MODE == MODE_SLICE || MODE == MODE_INJ) {
       if (MODE == MODE_SLICE && (w & 3) == 0)
         pickblk = philox4x32((uint32_t)chain, it32, (uint32_t)(w >> 2), RNG_PICK, k0, k1);
@@ -598,13 +663,13 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
     else if (FAM == HIPNS_FAM_GMM)
       L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
     else
-      L = lik_linreg<LPC, NJ>(up, nj, D, lr, tfm, gw, G, run.log_const);
+      L = lik_linreg<LPC, NJ>(up, nj, D, lr, tfm, wave, G, run.log_const);
 
     if (alive) {
       ++evals;
       if (MODE == MODE_INJ) {
         int64_t o = ((int64_t)chain * num_slices + s) * A.max_steps + k;
-        if (h == 0 && writer) {
+        if (h == 0) {
           A.trace_t[o] = t;
           A.trace_logL[o] = L;
         }
@@ -620,14 +685,12 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           alive = false;
           const int64_t o = run.chain_off + chain;
           float* dst = A.B.new_U + o * dps;
-          if (writer) {
 #pragma unroll
-            for (int j = 0; j < NJ; ++j)
-              if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
-            if (h == 0) {
-              A.B.new_logL[o] = logL0;
-              A.B.new_evals[o] = (int32_t)evals;
-            }
+          for (int j = 0; j < NJ; ++j)
+            if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+          if (h == 0) {
+            A.B.new_logL[o] = logL0;
+            A.B.new_evals[o] = (int32_t)evals;
           }
         }
       } else {
@@ -642,13 +705,13 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           ++cap_hits;
         }
         if (acc) {
-          if (MODE == MODE_INJ && h == 0 && writer) A.trace_steps[(int64_t)chain * num_slices + s] = k + 1;
+          if (MODE == MODE_INJ && h == 0) A.trace_steps[(int64_t)chain * num_slices + s] = k + 1;
 #pragma unroll
           for (int j = 0; j < NJ; ++j) U0[j] = up[j];
           logL0 = L;
           // phantom samples: x[-(k+1):-1] (uni_slice_sampler.py:432-440), evals 0
           int pidx = s - (num_slices - 1 - kph);
-          if (kph > 0 && pidx >= 0 && s < num_slices - 1 && writer) {
+          if (kph > 0 && pidx >= 0 && s < num_slices - 1) {
             if (MODE == MODE_INJ) {
               float* pd = A.ph_U + ((int64_t)chain * kph + pidx) * dps;
 #pragma unroll
@@ -674,14 +737,12 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
             alive = false;
             const int64_t o = run.chain_off + chain;
             float* dst = A.B.new_U + o * dps;
-            if (writer) {
 #pragma unroll
-              for (int j = 0; j < NJ; ++j)
-                if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
-              if (h == 0) {
-                A.B.new_logL[o] = logL0;
-                A.B.new_evals[o] = (int32_t)evals;
-              }
+            for (int j = 0; j < NJ; ++j)
+              if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+            if (h == 0) {
+              A.B.new_logL[o] = logL0;
+              A.B.new_evals[o] = (int32_t)evals;
             }
           } else {
             need_begin = true;
@@ -692,7 +753,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
       }
     }
   }
-  if (cap_hits > 0 && h == 0 && writer && MODE != MODE_INJ) {
+  if (cap_hits > 0 && h == 0 && MODE != MODE_INJ) {
     atomicAdd((unsigned long long*)&st->cap_hits, (unsigned long long)cap_hits);
     atomicOr(&st->flags, 1);
   }

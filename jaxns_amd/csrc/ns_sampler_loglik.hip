@@ -11,7 +11,6 @@ template <int LPC, int NJ, int FAM>
 __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
                                                 hipns_buffers_t Bf, int K, LinRegLds lrd) {
   constexpr int CW = HIPNS_WAVE / LPC;
-  constexpr bool SPLITK = (FAM == HIPNS_FAM_LINREG);
   extern __shared__ float lds[];
   const hipns_run_desc_t run = runs[0];
   const int dps = Bf.dps;
@@ -21,12 +20,12 @@ __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, doubl
   MvnLds mv;
   GmmLds gm;
   LinRegLds lr = lrd;
-  load_params(FAM, Bf.params + run.param_off, dps, D, K, lds, mv, gm, LPC, &lr);
   const int wpb = blockDim.x / HIPNS_WAVE;
+  load_params(FAM, Bf.params + run.param_off, dps, D, K, lds, mv, gm, LPC, &lr, wpb, NJ / 8);
   const int wave = threadIdx.x / HIPNS_WAVE;
   const int l = lane_id();
   const int c = l % CW, h = l / CW;
-  const int64_t row = SPLITK ? (int64_t)blockIdx.x * CW + c : ((int64_t)blockIdx.x * wpb + wave) * CW + c;
+  const int64_t row = ((int64_t)blockIdx.x * wpb + wave) * CW + c;
   float up[NJ];
   const bool ok = row < B;
 #pragma unroll
@@ -42,24 +41,33 @@ __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, doubl
     L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
   else
     L = lik_linreg<LPC, NJ>(up, nj, D, lr, tfm, wave, wpb, run.log_const);
-  if (ok && h == 0 && (!SPLITK || wave == 0)) out[row] = L;
+  if (ok && h == 0) out[row] = L;
 }
 
-template <int LPC, int FAM>
-static hipError_t launch_loglik_dp(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
-                                   const hipns_buffers_t& Bf, int K, int wpb, int dpmax, size_t lds, hipStream_t s,
+template <int LPC, int NJ, int FAM>
+static hipError_t launch_loglik_one(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
+                                   const hipns_buffers_t& Bf, int K, int wpb, size_t lds, hipStream_t s,
                                    const LinRegLds& lr) {
-  if (LPC == 2 && dpmax > 64) return launch_loglik_dp<4, FAM>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
   constexpr int CW = HIPNS_WAVE / LPC;
   int64_t groups = (B + CW - 1) / CW;
-  dim3 grid((unsigned)(FAM == HIPNS_FAM_LINREG ? groups : (groups + wpb - 1) / wpb)), block(wpb * HIPNS_WAVE);
-  if (dpmax <= 32)
-    hipLaunchKernelGGL((k_loglik<LPC, 32 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K, lr);
-  else if (dpmax <= 64)
-    hipLaunchKernelGGL((k_loglik<LPC, 64 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K, lr);
-  else if constexpr (LPC == 4)
-    hipLaunchKernelGGL((k_loglik<LPC, 128 / LPC, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K, lr);
+  dim3 grid((unsigned)((groups + wpb - 1) / wpb)), block(wpb * HIPNS_WAVE);
+  hipLaunchKernelGGL((k_loglik<LPC, NJ, FAM>), grid, block, lds, s, U, B, out, runs, Bf, K, lr);
   return hipGetLastError();
+}
+
+// the same instance choice as the sampler launchers (sampler_nj)
+template <int FAM>
+static hipError_t launch_loglik_fam(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
+                                   const hipns_buffers_t& Bf, int K, int lpc, int wpb, int dmax, size_t lds,
+                                   hipStream_t s, const LinRegLds& lr) {
+  if (lpc == 4 || dmax > 64) {
+    if (dmax <= 32) return launch_loglik_one<4, 8, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
+    if (dmax <= 64) return launch_loglik_one<4, 16, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
+    return launch_loglik_one<4, 32, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
+  }
+  if (dmax <= 16) return launch_loglik_one<2, 8, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
+  if (dmax <= 32) return launch_loglik_one<2, 16, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
+  return launch_loglik_one<2, 32, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
 }
 
 // adjoint statistics of the regression likelihood over a batch of points: logL, |r|^2 and y^T r
@@ -77,7 +85,7 @@ __global__ void __launch_bounds__(64) k_linreg_stats(const float* U, int64_t B, 
   MvnLds mv;
   GmmLds gm;
   LinRegLds lr = lrd;
-  load_params(HIPNS_FAM_LINREG, Bf.params + run.param_off, dps, D, 1, lds, mv, gm, LPC, &lr);
+  load_params(HIPNS_FAM_LINREG, Bf.params + run.param_off, dps, D, 1, lds, mv, gm, LPC, &lr, 1, NJ / 8);
   const int l = lane_id();
   const int c = l % CW, h = l / CW;
   const int64_t row = (int64_t)blockIdx.x * CW + c;
@@ -99,20 +107,22 @@ __global__ void __launch_bounds__(64) k_linreg_stats(const float* U, int64_t B, 
 }
 
 hipError_t launch_linreg_stats(const float* U, int64_t B, double* logL, double* rss, double* yr,
-                               const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int lpc, int dpmax,
+                               const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int lpc, int dmax,
                                size_t lds, hipStream_t s, const LinRegLds& lr) {
-  if (dpmax > 64) lpc = 4;
+  if (dmax > 64) lpc = 4;
   const int CW = HIPNS_WAVE / lpc;
   dim3 grid((unsigned)((B + CW - 1) / CW)), block(HIPNS_WAVE);
   if (lpc == 2) {
-    if (dpmax <= 32)
+    if (dmax <= 16)
+      hipLaunchKernelGGL((k_linreg_stats<2, 8>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
+    else if (dmax <= 32)
       hipLaunchKernelGGL((k_linreg_stats<2, 16>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
     else
       hipLaunchKernelGGL((k_linreg_stats<2, 32>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
   } else {
-    if (dpmax <= 32)
+    if (dmax <= 32)
       hipLaunchKernelGGL((k_linreg_stats<4, 8>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
-    else if (dpmax <= 64)
+    else if (dmax <= 64)
       hipLaunchKernelGGL((k_linreg_stats<4, 16>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
     else
       hipLaunchKernelGGL((k_linreg_stats<4, 32>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
@@ -121,19 +131,13 @@ hipError_t launch_linreg_stats(const float* U, int64_t B, double* logL, double* 
 }
 
 hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
-                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dpmax, size_t lds, hipStream_t s,
+                         const hipns_buffers_t& Bf, const hipns_launch_t& L, int dmax, size_t lds, hipStream_t s,
                          const LinRegLds& lr) {
   const int lpc = L.lanes_per_chain, wpb = L.waves_per_block, K = L.gmm_K;
-  if (L.family == HIPNS_FAM_GMM) {
-    if (lpc == 4) return launch_loglik_dp<4, HIPNS_FAM_GMM>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
-    return launch_loglik_dp<2, HIPNS_FAM_GMM>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
-  }
-  if (L.family == HIPNS_FAM_LINREG) {
-    if (lpc == 4) return launch_loglik_dp<4, HIPNS_FAM_LINREG>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
-    return launch_loglik_dp<2, HIPNS_FAM_LINREG>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
-  }
-  if (lpc == 4) return launch_loglik_dp<4, HIPNS_FAM_MVN>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
-  return launch_loglik_dp<2, HIPNS_FAM_MVN>(U, B, out, runs, Bf, K, wpb, dpmax, lds, s, lr);
+  if (L.family == HIPNS_FAM_GMM) return launch_loglik_fam<HIPNS_FAM_GMM>(U, B, out, runs, Bf, K, lpc, wpb, dmax, lds, s, lr);
+  if (L.family == HIPNS_FAM_LINREG)
+    return launch_loglik_fam<HIPNS_FAM_LINREG>(U, B, out, runs, Bf, K, lpc, wpb, dmax, lds, s, lr);
+  return launch_loglik_fam<HIPNS_FAM_MVN>(U, B, out, runs, Bf, K, lpc, wpb, dmax, lds, s, lr);
 }
 
 }  // namespace hipns

@@ -130,12 +130,12 @@ class DeviceBatch:
             prms.append(p)
             consts.append(c)
         poffs = np.cumsum([0] + [p.size for p in prms])
-        lr_ndata = lr_xs = lr_off = 0
+        lr_ndata = lr_panels = lr_off = 0
         if self.family == _lib.FAM_LINREG:
             data = fam.data
             if any(s.model.log_likelihood.data is not data for s in specs):
                 raise ValueError("all regression runs of a batch must share one RegressionData")
-            blk, lr_ndata, lr_xs = data.packed(dps)
+            blk, lr_ndata, lr_panels = data.packed(dps)
             lr_off = int(-(-poffs[-1] // 64) * 64)  # 256-byte aligned start of the shared rows
             prms.append(np.zeros(lr_off - poffs[-1], np.float32))
             prms.append(blk)
@@ -194,7 +194,7 @@ class DeviceBatch:
         self.bufs = b
         if lr_terms not in (3, 4):
             raise ValueError("lr_terms must be 3 (BF16x3) or 4")
-        self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, lr_ndata, max_steps, lr_xs,
+        self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, lr_ndata, max_steps, lr_panels,
                              lr_terms, lr_off)
         self._graph = None
         self._ev_ws = None

@@ -86,7 +86,7 @@ class LinRegAdjointStats:
         Up = np.full((self.M, self.dps), 0.5, np.float32)
         Up[:, :self.D] = U
         self.U = torch.from_numpy(Up).to(dev)
-        blk, npad, xs = family.data.packed(self.dps)
+        blk, npad, panels = family.data.packed(self.dps)
         self.n_run = 4 * self.dps + 4
         self.data_off = int(-(-self.n_run // 64) * 64)
         prm = np.zeros(self.data_off + blk.size, np.float32)
@@ -95,7 +95,7 @@ class LinRegAdjointStats:
         self.bufs = _lib.Buffers()
         self.bufs.params = _lib.ptr(self.params)
         self.bufs.dps = self.dps
-        self.launch = _lib.Launch(_lib.FAM_LINREG, lanes_per_chain, 1, 1, npad, 512, xs, lr_terms, self.data_off)
+        self.launch = _lib.Launch(_lib.FAM_LINREG, lanes_per_chain, 1, 1, npad, 512, panels, lr_terms, self.data_off)
         self.runs_host = (_lib.RunDesc * 1)()
         self.runs_dev = torch.zeros(C.sizeof(_lib.RunDesc), dtype=torch.uint8, device=dev)
         f64 = torch.float64

@@ -155,21 +155,25 @@ class RegressionData:
         self._packed = {}
 
     def packed(self, dps: int):
-        """[r0: npad][X hi: npad x xs bf16][X lo: npad x xs bf16][y: npad] as float32 words, npad, xs."""
+        """Shared device block as float32 words: [r0: npad][X hi][X lo][y: npad], with the column-scaled
+        X split into bf16 hi + lo parts stored as `panels` column panels of 16 ([panel][npad][16]);
+        returns (block, npad, panels)."""
         if dps in self._packed:
             return self._packed[dps]
         npad = -(-self.N // 32) * 32
-        xs = -(-dps // 32) * 32
+        # kernel instances read 32, 64 or 128 columns (the dimension class of D), zero beyond D
+        panels = 2 if self.D <= 32 else (4 if self.D <= 64 else 8)
         r0 = np.zeros(npad, np.float32)
         r0[:self.N] = self.y - self.X @ self.w0
-        Xs = np.zeros((npad, xs), np.float32)
+        Xs = np.zeros((npad, 16 * panels), np.float32)
         Xs[:self.N, :self.D] = self.X / self.col_scale
         hi = _bf16_rne(Xs)
         lo = _bf16_rne(Xs - _bf16_to_f32(hi))
+        to_panels = lambda a: np.ascontiguousarray(a.reshape(npad, panels, 16).transpose(1, 0, 2)).ravel()
         yp = np.zeros(npad, np.float32)
         yp[:self.N] = self.y
-        blk = np.concatenate([r0, hi.ravel().view(np.float32), lo.ravel().view(np.float32), yp])
-        self._packed[dps] = (blk, npad, xs)
+        blk = np.concatenate([r0, to_panels(hi).view(np.float32), to_panels(lo).view(np.float32), yp])
+        self._packed[dps] = (blk, npad, panels)
         return self._packed[dps]
 
 

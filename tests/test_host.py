@@ -145,12 +145,14 @@ def test_linreg_device_packing_reproduces_likelihood():
         model, _, lik = linreg_problem(300, 20, seed=4, sigma=0.3, tau=1.5, scaled_cols=scaled)
         D, dps = 20, 32
         prm, c = lik.device_params(model, dps)
-        blk, npad, xs = lik.data.packed(dps)
-        assert npad == 320 and xs == 32 and blk.size == 2 * npad + npad * xs
-        np.testing.assert_array_equal(blk[npad + npad * xs:npad + npad * xs + 300], lik.y.astype(np.float32))
+        blk, npad, panels = lik.data.packed(dps)
+        assert npad == 320 and panels == 2 and blk.size == 2 * npad + npad * 16 * panels
+        ne = npad * 16 * panels  # bf16 elements of one X part
+        np.testing.assert_array_equal(blk[npad + ne:npad + ne + 300], lik.y.astype(np.float32))
         r0 = blk[:npad].astype(np.float64)
-        hi = blk[npad:npad + npad * xs // 2].view(np.uint16).reshape(npad, xs)
-        lo = blk[npad + npad * xs // 2:npad + npad * xs].view(np.uint16).reshape(npad, xs)
+        from_panels = lambda a: a.reshape(panels, npad, 16).transpose(1, 0, 2).reshape(npad, 16 * panels)
+        hi = from_panels(blk[npad:npad + ne // 2].view(np.uint16))
+        lo = from_panels(blk[npad + ne // 2:npad + ne].view(np.uint16))
         Xh, Xl = _bf16_to_f32(hi).astype(np.float64), _bf16_to_f32(lo).astype(np.float64)
         p, s_, w0 = prm[dps:2 * dps], prm[2 * dps:3 * dps], prm[3 * dps:4 * dps]
         inv2s2 = float(prm[4 * dps:4 * dps + 2].view(np.float64)[0])
