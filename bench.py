@@ -164,6 +164,19 @@ class LinRegWorkload(Workload):
         return linreg_oracle_spec(self.lik, self.tau)
 
 
+class LinRegScaleWorkload(LinRegWorkload):
+    """Precision at scale: 65536 rows x 128 features, 10 % of the columns x 100, sigma=0.1,
+    w ~ N(0, I); num_live_points=4096, s=3."""
+
+    def __init__(self):
+        from problems import linreg_problem
+        self.Ndata, self.D, self.N, self.s, self.tau = 65536, 128, 4096, 3, 1.0
+        self.lr_terms = 3
+        self.name = ("Bayesian linear regression 65536 x 128 (10% of columns x100), sigma=0.1, w ~ N(0, I), "
+                     "num_live_points=4096, s=3 (BF16x3 likelihood GEMM, FP64 evidence)")
+        self.model, self.log_Z, self.lik = linreg_problem(65536, 128, seed=0, sigma=0.1, tau=1.0, scaled_cols=0.1)
+
+
 def make_workload(name):
     if name == "c1":
         return MvnWorkload(10, 300, 5)
@@ -173,13 +186,17 @@ def make_workload(name):
         return GmmBatchWorkload()
     if name == "c4":
         return LinRegWorkload()
+    if name == "c5":
+        return LinRegScaleWorkload()
     raise ValueError(name)
 
 
-def make_batch(wl, seeds, lpc, wpb):
+def make_batch(wl, seeds, lpc, wpb, plain=False):
+    """plain=True: the plain GPU form's arithmetic (FP32 GEMM for the regression likelihood)."""
     from jaxns_amd.engine import DeviceBatch
     specs, truths = wl.specs(seeds)
-    return DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=wpb, max_steps=512, lr_terms=wl.lr_terms), truths
+    terms = 1 if (plain and isinstance(wl, LinRegWorkload)) else wl.lr_terms
+    return DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=wpb, max_steps=512, lr_terms=terms), truths
 
 
 def term_for(batch):
@@ -273,13 +290,67 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def precision_at_scale(wl, R, shells, lpc, wpb, steps):
+    """Partial runs of the 65536 x 128 regression (a full run needs ~1e9 likelihood evaluations):
+    `shells` shells of R runs in each product form, timed with HIP events around the sampling kernel,
+    plus the logL error of the accepted points against the FP64 likelihood on the host."""
+    import torch
+    from jaxns_amd._lib import check, lib, make_term_cond, ptr, stream_ptr
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    out = {}
+    for terms, label in ((3, "bf16x3"), (1, "fp32")):
+        specs = [RunSpec(model=wl.model, num_live_points=wl.N, num_slices=wl.D * wl.s, k_phantom=0,
+                         max_samples=wl.N * (shells + 4), seed=sd) for sd in range(R)]
+        b = DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=wpb, max_steps=512, lr_terms=terms)
+        b.init_state()
+        tc = make_term_cond(max_samples=float(specs[0].max_samples))
+        s = stream_ptr()
+        durs, evs = [], []
+        prev = sum(st.num_likelihood_evaluations for st in b.states())
+        errs = []
+        for k in range(shells):
+            check(lib().hipns_shell_discard(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host, C.byref(b.bufs), s))
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            check(lib().hipns_shell_sample(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host, C.byref(b.bufs),
+                                           C.byref(b.launch), s))
+            e1.record()
+            torch.cuda.synchronize()
+            # accepted points of run 0 against the FP64 likelihood (prior transform in FP64 as well)
+            n0 = specs[0].shell
+            U = b.new_U[:n0, :wl.D].double().cpu().numpy()
+            Ld = b.new_logL[:n0].cpu().numpy()
+            sel = np.linspace(0, n0 - 1, 64).astype(int)
+            from scipy.special import ndtri
+            Lh = wl.lik.log_prob_numpy(wl.tau * ndtri(U[sel]))
+            errs.append(np.abs(Ld[sel] - Lh) / np.abs(Lh))
+            check(lib().hipns_shell_replace(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host, C.byref(b.bufs), s))
+            check(lib().hipns_shell_register(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host,
+                                             C.byref(b.bufs), C.byref(tc), s))
+            torch.cuda.synchronize()
+            cur = sum(st.num_likelihood_evaluations for st in b.states())
+            durs.append(e0.elapsed_time(e1) * 1e-3)
+            evs.append(cur - prev)
+            prev = cur
+            log(f"{label} shell {k}: {durs[-1] * 1e3:.1f} ms, {evs[-1]} evals")
+        durs, evs = np.array(durs), np.array(evs, dtype=np.float64)
+        eps = float(evs.sum() / durs.sum())
+        e = np.concatenate(errs)
+        out[label] = dict(evals_per_s=eps, avg_launch_ms=float(durs.mean() * 1e3), evals_per_launch=float(evs.mean()),
+                          logL_rel_err_max=float(e.max()), logL_rel_err_median=float(np.median(e)),
+                          achieved_TFLOPs=eps * wl.work_per_eval(None) / 1e12)
+        del b
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--runs", type=int, default=None, help="runs per step (chosen form); default 128, c3/c4: 256")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--shells", type=int, default=3, help="c5: shells per partial run")
     ap.add_argument("--graph-iters", type=int, default=8)
     ap.add_argument("--lpc", type=int, default=4)
     ap.add_argument("--wpb", type=int, default=4)
@@ -301,8 +372,26 @@ def main():
     wl = make_workload(args.config)
     if args.config == "c4":
         wl.lr_terms = args.lr_terms
-    R = args.runs or (256 if args.config in ("c3", "c4") else 128)
+    R = args.runs or (256 if args.config in ("c3", "c4") else (8 if args.config == "c5" else 128))
     seed_base = 0
+    if args.config == "c5":
+        t0 = time.perf_counter()
+        res = precision_at_scale(wl, R, args.shells, 4, args.wpb, args.steps)
+        wall = time.perf_counter() - t0
+        bf, fp = res["bf16x3"], res["fp32"]
+        roof = dict(bound="mfma", achieved=bf["achieved_TFLOPs"], peak=PEAK_BF16_MFMA / 1e12, unit="TFLOP/s",
+                    frac=bf["achieved_TFLOPs"] * 1e12 / PEAK_BF16_MFMA, traffic=None, kernel=wl.kernel,
+                    avg_launch_ms=bf["avg_launch_ms"], evals_per_launch=bf["evals_per_launch"])
+        print(json.dumps({
+            "metric": "likelihood evals/s of partial runs (BF16x3 vs plain FP32), 1 GPU", "value": bf["evals_per_s"],
+            "unit": "evals/s", "n_gpus": 1, "steps": args.shells, "warmup": 0, "ms_per_step": bf["avg_launch_ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16x3 likelihood GEMM (MFMA, f32 accumulate)", "data": "synthetic (seeded numpy)",
+            "config": {"workload": wl.name, "runs_per_step": R, "shells": args.shells, "lanes_per_chain": 4,
+                       "waves_per_block": args.wpb},
+            "bf16x3": bf, "plain_fp32": fp, "bf16x3_vs_fp32": bf["evals_per_s"] / fp["evals_per_s"],
+            "log_Z_true": float(wl.log_Z), "wall_s": wall, "roofline": roof, "cpu_baseline": None}), flush=True)
+        return
     if args.profile_pass:
         if args.copy_calib:
             n = args.copy_calib * (1 << 20) // 4
@@ -361,11 +450,11 @@ def main():
     dz = logZ[:, 0] - truths
     evals_per_run = evals_total / (R * args.steps)
 
-    # ---------------- plain form: one run at a time, host sync per shell
+    # ---------------- plain form: one run at a time, host sync per shell, FP32 likelihood GEMM
     tp = 0.0
     pz = []
     for k in range(args.plain_runs):
-        b, tr = make_batch(wl, [seed_base + k], args.lpc, args.wpb)
+        b, tr = make_batch(wl, [seed_base + k], args.lpc, args.wpb, plain=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         pz.append(run_batch(b, 0)[0][0] - tr[0])

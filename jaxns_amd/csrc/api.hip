@@ -76,7 +76,8 @@ int check_launch(const hipns_launch_t* L) {
     if (L->lr_panels < 2 || L->lr_panels % 2 || L->lr_panels > HIPNS_MAX_DIM / 16)
       return fail(HIPNS_EINVAL, "lr_panels must be even, 2..%d", HIPNS_MAX_DIM / 16);
     if (L->lr_data_off < 0 || L->lr_data_off % 4) return fail(HIPNS_EINVAL, "lr_data_off must be a multiple of 4");
-    if (L->lr_terms != 0 && L->lr_terms != 3 && L->lr_terms != 4) return fail(HIPNS_EINVAL, "lr_terms must be 3 or 4");
+    if (L->lr_terms != 0 && L->lr_terms != 1 && L->lr_terms != 3 && L->lr_terms != 4)
+      return fail(HIPNS_EINVAL, "lr_terms must be 1, 3 or 4");
   }
   if (L->lanes_per_chain != 2 && L->lanes_per_chain != 4)
     return fail(HIPNS_EINVAL, "lanes_per_chain must be 2 or 4");
@@ -88,7 +89,7 @@ int check_launch(const hipns_launch_t* L) {
 }
 
 // device view of the shared regression rows: params + lr_data_off = [r0: npad]
-// [X hi: panels x npad x 16 bf16][X lo: same][y: npad]
+// [X hi: panels x npad x 16 bf16][X lo: same][y: npad][X: panels x npad x 16 f32]
 hipns::LinRegLds linreg_view(const hipns_buffers_t* bufs, const hipns_launch_t* L) {
   hipns::LinRegLds lr{};
   const float* base = bufs->params + L->lr_data_off;
@@ -97,9 +98,10 @@ hipns::LinRegLds linreg_view(const hipns_buffers_t* bufs, const hipns_launch_t* 
   lr.xh = (const uint16_t*)(base + npad);
   lr.xl = lr.xh + xe;
   lr.y = base + npad + xe;
+  lr.xf = base + 2 * npad + xe;
   lr.npad = L->lr_ndata;
   lr.panels = L->lr_panels;
-  lr.terms = L->lr_terms == 4 ? 4 : 3;
+  lr.terms = (L->lr_terms == 4 || L->lr_terms == 1) ? L->lr_terms : 3;
   return lr;
 }
 
@@ -130,6 +132,7 @@ int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A
     A.lr_r0 = lr.r0;
     A.lr_xh = lr.xh;
     A.lr_xl = lr.xl;
+    A.lr_xf = lr.xf;
     A.lr_npad = lr.npad;
     A.lr_panels = lr.panels;
     A.lr_terms = lr.terms;

@@ -25,6 +25,7 @@ struct SampleArgs {
   const float* lr_r0;
   const uint16_t* lr_xh;
   const uint16_t* lr_xl;
+  const float* lr_xf;
   int32_t lr_npad;
   int32_t lr_panels;
   int32_t lr_terms;
@@ -58,8 +59,9 @@ struct LinRegLds {
   const uint16_t* xh;  // [panels][npad][16] bf16 hi part of the column-scaled X
   const uint16_t* xl;  // [panels][npad][16] bf16 lo part (X - hi)
   const float* y;      // [npad] targets (zero on padding rows), read by the adjoint statistics only
+  const float* xf;     // [panels][npad][16] FP32 column-scaled X (the plain FP32 form)
   int npad, panels;
-  int terms;           // 3: BF16x3; 4: adds X hi times the third bf16 part of w - w0
+  int terms;           // 1: plain FP32 MFMA; 3: BF16x3; 4: adds X hi times the third bf16 part of w - w0
 };
 
 // register slots per lane of the dispatched kernel instance for runs of at most dmax dimensions

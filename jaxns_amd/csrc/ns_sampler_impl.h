@@ -139,33 +139,44 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
   const int m = l % CW;
   const int h = l / CW;
   const bool t4 = P.terms == 4;
-  (void)nj;
-  // publish this wavefront's B operands: [G][NG][3][64] vectors of 8 bf16
-  bf16x8_t* Bs = (bf16x8_t*)P.bops;
+  const bool fp32 = P.terms == 1;
+  float dwv[NJ];
 #pragma unroll
-  for (int gg = 0; gg < NG; ++gg) {
-    bf16x8_t bh, bl, bt;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int j = gg * 8 + i;
-      const int d = owned_dim<LPC>(h, j);
-      float dw = 0.0f;
-      if (j < nj && d < D) {
-        float u = up[j];
-        float q = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
-        dw = fmaf(P.s[d], q, P.p[d]) - P.w0[d];
-      }
-      // published negated, so that the MFMA chain started from C = r0 yields r = r0 - X (w - w0)
-      const __bf16 hi = (__bf16)dw;
-      const float rem = dw - (float)hi;
-      const __bf16 lo = (__bf16)rem;
-      bh[i] = -hi;
-      bl[i] = -lo;
-      bt[i] = -(__bf16)(rem - (float)lo);
+  for (int j = 0; j < NJ; ++j) {
+    const int d = owned_dim<LPC>(h, j);
+    float dw = 0.0f;
+    if (j < nj && d < D) {
+      float u = up[j];
+      float q = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
+      dw = fmaf(P.s[d], q, P.p[d]) - P.w0[d];
     }
-    Bs[((g * NG + gg) * 3 + 0) * HIPNS_WAVE + l] = bh;
-    Bs[((g * NG + gg) * 3 + 1) * HIPNS_WAVE + l] = bl;
-    Bs[((g * NG + gg) * 3 + 2) * HIPNS_WAVE + l] = bt;
+    dwv[j] = dw;
+  }
+  // publish this wavefront's B operands, negated so that the MFMA chain started from C = r0 yields
+  // r = r0 - X (w - w0): BF16 forms [G][NG][3][64] vectors of 8 bf16, plain FP32 [G][NJ][64] floats
+  bf16x8_t* Bs = (bf16x8_t*)P.bops;
+  float* Bf = (float*)P.bops;
+  if (fp32) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) Bf[(g * NJ + j) * HIPNS_WAVE + l] = -dwv[j];
+  } else {
+#pragma unroll
+    for (int gg = 0; gg < NG; ++gg) {
+      bf16x8_t bh, bl, bt;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const float dw = dwv[gg * 8 + i];
+        const __bf16 hi = (__bf16)dw;
+        const float rem = dw - (float)hi;
+        const __bf16 lo = (__bf16)rem;
+        bh[i] = -hi;
+        bl[i] = -lo;
+        bt[i] = -(__bf16)(rem - (float)lo);
+      }
+      Bs[((g * NG + gg) * 3 + 0) * HIPNS_WAVE + l] = bh;
+      Bs[((g * NG + gg) * 3 + 1) * HIPNS_WAVE + l] = bl;
+      Bs[((g * NG + gg) * 3 + 2) * HIPNS_WAVE + l] = bt;
+    }
   }
   __syncthreads();
   const int tiles = P.npad / CW;
@@ -179,6 +190,68 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
   constexpr int NR = (LPC == 2) ? 16 : 4;  // accumulator rows per lane
   // rows of tile tl held by this lane (the MFMA output layout)
   auto tile_row = [&](int tl, int i) { return (LPC == 2) ? tl * 32 + 8 * (i >> 2) + 4 * h + (i & 3) : tl * 16 + 4 * h + i; };
+  if (fp32) {
+    // plain FP32 form: v_mfma_f32_32x32x2_f32 (LPC=2) / v_mfma_f32_16x16x4_f32 (LPC=4), one MFMA per
+    // owned dimension slot, X read in FP32 from its own panels
+    for (int tl = t0; tl < t1; ++tl) {
+      const int row = tl * CW + m;
+      float xa[NJ];
+#pragma unroll
+      for (int gg = 0; gg < NG; ++gg) {
+        const float* xp = (LPC == 2) ? P.xf + ((int64_t)gg * P.npad + row) * 16 + 8 * h
+                                     : P.xf + ((int64_t)(2 * gg + (h >> 1)) * P.npad + row) * 16 + 8 * (h & 1);
+        const float4 v0 = *(const float4*)xp, v1 = *(const float4*)(xp + 4);
+        xa[gg * 8 + 0] = v0.x;
+        xa[gg * 8 + 1] = v0.y;
+        xa[gg * 8 + 2] = v0.z;
+        xa[gg * 8 + 3] = v0.w;
+        xa[gg * 8 + 4] = v1.x;
+        xa[gg * 8 + 5] = v1.y;
+        xa[gg * 8 + 6] = v1.z;
+        xa[gg * 8 + 7] = v1.w;
+      }
+      float rv[NR], yv[NR];
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        rv[i] = P.r0[tile_row(tl, i)];
+        if (STATS) yv[i] = P.y[tile_row(tl, i)];
+      }
+#pragma unroll
+      for (int q = 0; q < LR_MAX_G; ++q) {
+        if (q < G) {
+          const float* B = Bf + q * NJ * HIPNS_WAVE + l;
+          float part = 0.0f, ypart = 0.0f;
+          if constexpr (LPC == 2) {
+            f32x16 acc;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[i] = rv[i];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+              acc = __builtin_amdgcn_mfma_f32_32x32x2f32(xa[j], B[j * HIPNS_WAVE], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              part = fmaf(acc[i], acc[i], part);
+              if (STATS) ypart = fmaf(yv[i], acc[i], ypart);
+            }
+          } else {
+            f32x4 acc;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] = rv[i];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[j], B[j * HIPNS_WAVE], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              part = fmaf(acc[i], acc[i], part);
+              if (STATS) ypart = fmaf(yv[i], acc[i], ypart);
+            }
+          }
+          ss[q] += (double)part;
+          if (STATS) sy[q] += (double)ypart;
+        }
+      }
+    }
+  } else {
   // software pipeline: the X panels and r0 of the next tile are in flight while this one multiplies.
   // All NG k-groups are read: the panels cover NJ * LPC columns, the unused ones are zero.
   bf16x8_t ah[NG], al[NG], nh[NG], nl[NG];
@@ -253,6 +326,7 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
     }
 #pragma unroll
     for (int i = 0; i < NR; ++i) r0v[i] = nr0[i];
+  }
   }
   // partial sums of every group from this wavefront's rows -> LDS [group][wavefront][chain]
 #pragma unroll
@@ -483,6 +557,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
     lr.r0 = A.lr_r0;
     lr.xh = A.lr_xh;
     lr.xl = A.lr_xl;
+    lr.xf = A.lr_xf;
     lr.y = nullptr;
     lr.npad = A.lr_npad;
     lr.panels = A.lr_panels;

@@ -192,8 +192,8 @@ class DeviceBatch:
         b.params = ptr(self.params)
         b.dps = dps
         self.bufs = b
-        if lr_terms not in (3, 4):
-            raise ValueError("lr_terms must be 3 (BF16x3) or 4")
+        if lr_terms not in (1, 3, 4):
+            raise ValueError("lr_terms must be 1 (plain FP32), 3 (BF16x3) or 4")
         self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, lr_ndata, max_steps, lr_panels,
                              lr_terms, lr_off)
         self._graph = None

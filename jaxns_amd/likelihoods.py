@@ -155,9 +155,9 @@ class RegressionData:
         self._packed = {}
 
     def packed(self, dps: int):
-        """Shared device block as float32 words: [r0: npad][X hi][X lo][y: npad], with the column-scaled
-        X split into bf16 hi + lo parts stored as `panels` column panels of 16 ([panel][npad][16]);
-        returns (block, npad, panels)."""
+        """Shared device block as float32 words: [r0: npad][X hi][X lo][y: npad][X], with the
+        column-scaled X split into bf16 hi + lo parts and also kept in FP32 (the plain form), each stored
+        as `panels` column panels of 16 ([panel][npad][16]); returns (block, npad, panels)."""
         if dps in self._packed:
             return self._packed[dps]
         npad = -(-self.N // 32) * 32
@@ -172,7 +172,8 @@ class RegressionData:
         to_panels = lambda a: np.ascontiguousarray(a.reshape(npad, panels, 16).transpose(1, 0, 2)).ravel()
         yp = np.zeros(npad, np.float32)
         yp[:self.N] = self.y
-        blk = np.concatenate([r0, to_panels(hi).view(np.float32), to_panels(lo).view(np.float32), yp])
+        blk = np.concatenate([r0, to_panels(hi).view(np.float32), to_panels(lo).view(np.float32), yp,
+                              to_panels(Xs)])
         self._packed[dps] = (blk, npad, panels)
         return self._packed[dps]
 

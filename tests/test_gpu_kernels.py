@@ -184,7 +184,7 @@ def test_batched_loglik_vs_oracle(fam, D, lpc, wpb, nd):
     Up[:, :D] = U
     Ud = torch.from_numpy(Up).cuda()
     ref = spec.log_likelihood(U.astype(np.float64))
-    for terms in ((3, 4) if fam == "linreg" else (0,)):
+    for terms in ((1, 3, 4) if fam == "linreg" else (0,)):
         out = torch.empty(B, dtype=torch.float64, device="cuda")
         b.launch.lr_terms = terms
         check(lib().hipns_log_likelihood(ptr(Ud), B, ptr(out), ptr(b.runs_dev), b.runs_host, C.byref(b.bufs),
@@ -193,7 +193,7 @@ def test_batched_loglik_vs_oracle(fam, D, lpc, wpb, nd):
         rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
         say("terms", terms, "max rel err", rel.max())
         # BF16x3 on prior draws: the 16-bit split of w - w0 is shared by all rows, 3e-5 relative;
-        # every other form: 1e-5 relative
+        # every other form (plain FP32 = 1, four products = 4, MVN, GMM): 1e-5 relative
         assert rel.max() < (3e-5 if terms == 3 else 1e-5)
         if fam == "linreg":  # posterior draws: the centred residual keeps the error far below the posterior scale
             post = np.abs(got - ref)[B // 2:]

@@ -146,8 +146,10 @@ def test_linreg_device_packing_reproduces_likelihood():
         D, dps = 20, 32
         prm, c = lik.device_params(model, dps)
         blk, npad, panels = lik.data.packed(dps)
-        assert npad == 320 and panels == 2 and blk.size == 2 * npad + npad * 16 * panels
+        assert npad == 320 and panels == 2 and blk.size == 2 * npad + 2 * npad * 16 * panels
         ne = npad * 16 * panels  # bf16 elements of one X part
+        Xf = blk[2 * npad + ne:].reshape(panels, npad, 16).transpose(1, 0, 2).reshape(npad, 16 * panels)
+        np.testing.assert_allclose(Xf[:300, :D], lik.X / lik.data.col_scale, rtol=1e-7)
         np.testing.assert_array_equal(blk[npad + ne:npad + ne + 300], lik.y.astype(np.float32))
         r0 = blk[:npad].astype(np.float64)
         from_panels = lambda a: a.reshape(panels, npad, 16).transpose(1, 0, 2).reshape(npad, 16 * panels)
