@@ -628,7 +628,10 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
     if (MODE == MODE_INJ) {
       const float* dsrc = A.inj_dirs + ((int64_t)chain * num_slices + si) * dps;
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) nv[j] = (j < nj) ? dsrc[owned_dim<LPC>(h, j)] : 0.0f;
+      for (int j = 0; j < NJ; ++j) {
+        const int d = owned_dim<LPC>(h, j);
+        nv[j] = (j < nj && d < D) ? dsrc[d] : 0.0f;
+      }
     } else if (D == 1) {
 #pragma unroll
       for (int j = 0; j < NJ; ++j) nv[j] = (owned_dim<LPC>(h, j) == 0) ? 1.0f : 0.0f;
@@ -668,11 +671,11 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
 #pragma unroll
       for (int j = 0; j < NJ; ++j) nv[j] = nv[j] * inv;
     }
-    // _slice_bounds (uni_slice_sampler.py:41-64)
+    // _slice_bounds (uni_slice_sampler.py:41-64).  Padding slots (beyond D or nj) hold U0 = 0.5 and
+    // n = +0, whose quotients are +-inf and never the bound, so every slot is taken unconditionally.
     float rmin = __builtin_inff(), lmax = -__builtin_inff();
 #pragma unroll
-    for (int j = 0; j < NJ; ++j)
-      if (j < nj && owned_dim<LPC>(h, j) < D) bound_candidates(U0[j], nv[j], rmin, lmax);
+    for (int j = 0; j < NJ; ++j) bound_candidates(U0[j], nv[j], rmin, lmax);
     right = chain_min<LPC>(rmin);
     left = chain_max<LPC>(lmax);
   };
