@@ -243,6 +243,45 @@ int hipns_argsort_stable(const double* keys_dev, int64_t n, int64_t* perm_dev, v
 size_t hipns_argsort_workspace_size(int64_t n);
 
 /* ---------------------------------------------------------------------
+ * Likelihoods evaluated by the caller (a torch callable): the same shell,
+ * one step per call.  Chain state lives in caller-owned arrays of C rows
+ * (C >= the sum of num_live_points; chain i of run r is row chain_off_r + i).
+ * ------------------------------------------------------------------- */
+typedef struct hipns_ext_state {
+  float* U0;          /* [C][dps] current point of each chain */
+  float* nv;          /* [C][dps] slice direction */
+  float* prop_U;      /* [C][dps] proposal of the current step; rows of finished chains hold 0.5 */
+  double* logL0;      /* [C] */
+  double* prop_logL;  /* [C] written by the caller: Model.forward of prop_U (NaN is read as -inf) */
+  float* left;        /* [C] bracket */
+  float* right;
+  float* t;           /* [C] last pick */
+  int32_t* s;         /* [C] slice index */
+  int32_t* k;         /* [C] shrinks of the current slice */
+  int32_t* evals;     /* [C] likelihood evaluations of the chain */
+  int32_t* flags;     /* [C] bit 0 running, bit 1 begins a slice, bit 2 shrinks */
+} hipns_ext_state_t;
+
+/* One step of every chain of the batch (UniDimSliceSampler.get_sample_from_seed,
+ * uni_slice_sampler.py:360-441, UniformSampler uniform_samplers.py:42-85, or the initial
+ * uniform draws, initialisation.py:20-84): step 0 seeds the chains and writes their first
+ * proposals; step w > 0 first consumes prop_logL of step w-1 (acceptance, shrink, phantoms,
+ * outputs into new_U/new_logL/new_evals), then writes the proposals of step w.  mode: 0 slice
+ * shell, 1 uniform-sampler shell, 2 initial live points.  A shell is hipns_shell_discard, steps
+ * until no chain runs (flags bit 0), hipns_shell_replace and hipns_shell_register; the initial
+ * state is hipns_init_prepare, mode-2 steps, hipns_init_finalize. */
+int hipns_ext_step(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                   const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                   const hipns_ext_state_t* ext, int32_t mode, int32_t step, void* stream);
+
+/* The parts of hipns_create_init_state around its sampling (initialisation.py:20-108): register
+ * and sample-collection reset, then the initial sort of the live points. */
+int hipns_init_prepare(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                       const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream);
+int hipns_init_finalize(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream);
+
+/* ---------------------------------------------------------------------
  * (b) batched prior transform and likelihood
  * ------------------------------------------------------------------- */
 

@@ -69,6 +69,11 @@ class Launch(C.Structure):
                 ("lr_data_off", _i64)]
 
 
+class ExtState(C.Structure):
+    _fields_ = [("U0", _vp), ("nv", _vp), ("prop_U", _vp), ("logL0", _vp), ("prop_logL", _vp), ("left", _vp),
+                ("right", _vp), ("t", _vp), ("s", _vp), ("k", _vp), ("evals", _vp), ("flags", _vp)]
+
+
 assert C.sizeof(RunDesc) == 88, C.sizeof(RunDesc)
 assert C.sizeof(Launch) == 40, C.sizeof(Launch)
 
@@ -120,6 +125,9 @@ def _declare(L):
     L.hipns_argsort_workspace_size.restype = C.c_size_t
     L.hipns_argsort_workspace_size.argtypes = [_i64]
     L.hipns_log_likelihood.argtypes = [_vp, _i64, _vp, _vp, P(RunDesc), P(Buffers), P(Launch), _vp]
+    L.hipns_ext_step.argtypes = [_vp, _vp, _i32, P(RunDesc), P(Buffers), P(Launch), P(ExtState), _i32, _i32, _vp]
+    L.hipns_init_prepare.argtypes = [_vp, _vp, _i32, P(RunDesc), P(Buffers), _vp]
+    L.hipns_init_finalize.argtypes = [_vp, _vp, _i32, P(RunDesc), P(Buffers), _vp]
     L.hipns_batch_evidence.argtypes = [_vp, _vp, _i32, P(RunDesc), P(Buffers), _vp, _vp, _vp, C.c_size_t, _vp]
     L.hipns_batch_evidence_workspace_size.restype = C.c_size_t
     L.hipns_batch_evidence_workspace_size.argtypes = [_i32, P(RunDesc)]
@@ -137,7 +145,8 @@ EXPORTED = ["hipns_abi_version", "hipns_last_error", "hipns_create_init_state", 
             "hipns_evidence_scan", "hipns_evidence_scan_workspace_size", "hipns_count_crossed_edges",
             "hipns_count_crossed_edges_workspace_size", "hipns_argsort_stable", "hipns_argsort_workspace_size",
             "hipns_log_likelihood", "hipns_linreg_stats", "hipns_batch_evidence",
-            "hipns_batch_evidence_workspace_size", "hipns_prior_transform", "hipns_slice_chains_injected", "hipns_philox4x32",
+            "hipns_batch_evidence_workspace_size", "hipns_ext_step", "hipns_init_prepare", "hipns_init_finalize",
+            "hipns_prior_transform", "hipns_slice_chains_injected", "hipns_philox4x32",
             "hipns_ndtri_f32"]
 
 

@@ -230,13 +230,7 @@ int hipns_create_init_state(const hipns_run_desc_t* runs_dev, hipns_run_state_t*
   if ((rc = check_launch(launch))) return rc;
   if (!runs_dev || !states_dev) return fail(HIPNS_EINVAL, "null device run/state arrays");
   hipStream_t s = as_stream(stream);
-  int64_t max_ms = 0;
-  for (int r = 0; r < R; ++r) max_ms = runs_host[r].max_samples > max_ms ? runs_host[r].max_samples : max_ms;
-  hipLaunchKernelGGL(hipns::k_init_state, dim3((R + 63) / 64), dim3(64), 0, s, states_dev, R);
-  HIPNS_CHECK_LAUNCH("k_init_state");
-  hipLaunchKernelGGL(hipns::k_init_samples, dim3(rows_blocks(max_ms, bufs->dps), R), dim3(256), 0, s, runs_dev, R,
-                     *bufs);
-  HIPNS_CHECK_LAUNCH("k_init_samples");
+  if ((rc = hipns_init_prepare(runs_dev, states_dev, R, runs_host, bufs, stream))) return rc;
   hipns::SampleArgs A = {};
   A.runs = runs_dev;
   A.states = states_dev;
@@ -244,12 +238,67 @@ int hipns_create_init_state(const hipns_run_desc_t* runs_dev, hipns_run_state_t*
   A.max_steps = launch->max_steps;
   A.gmm_K = launch->gmm_K;
   if ((rc = sampler_launch(hipns::MODE_INIT, launch, A, R, maxN, maxD, bufs->dps, 0, s))) return rc;
-  size_t lds = hipns::replace_lds_bytes(maxN);
-  if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "N=%d needs %zu B LDS for the sort", maxN, lds);
-  hipLaunchKernelGGL(hipns::k_replace, dim3(R), dim3(1024), lds, s, runs_dev, states_dev, *bufs, 1);
-  HIPNS_CHECK_LAUNCH("k_replace(init)");
   (void)workspace;
   (void)workspace_bytes;
+  return hipns_init_finalize(runs_dev, states_dev, R, runs_host, bufs, stream);
+}
+
+int hipns_init_prepare(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                       const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  int maxD, maxN, maxS;
+  if ((rc = check_runs(runs_host, R, bufs, &maxD, &maxN, &maxS))) return rc;
+  if (!runs_dev || !states_dev) return fail(HIPNS_EINVAL, "null device run/state arrays");
+  hipStream_t s = as_stream(stream);
+  int64_t max_ms = 0;
+  for (int r = 0; r < R; ++r) max_ms = runs_host[r].max_samples > max_ms ? runs_host[r].max_samples : max_ms;
+  hipLaunchKernelGGL(hipns::k_init_state, dim3((R + 63) / 64), dim3(64), 0, s, states_dev, R);
+  HIPNS_CHECK_LAUNCH("k_init_state");
+  hipLaunchKernelGGL(hipns::k_init_samples, dim3(rows_blocks(max_ms, bufs->dps), R), dim3(256), 0, s, runs_dev, R,
+                     *bufs);
+  HIPNS_CHECK_LAUNCH("k_init_samples");
+  return HIPNS_OK;
+}
+
+int hipns_init_finalize(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                        const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  int maxD, maxN, maxS;
+  if ((rc = check_runs(runs_host, R, bufs, &maxD, &maxN, &maxS))) return rc;
+  if (!runs_dev || !states_dev) return fail(HIPNS_EINVAL, "null device run/state arrays");
+  size_t lds = hipns::replace_lds_bytes(maxN);
+  if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "N=%d needs %zu B LDS for the sort", maxN, lds);
+  hipLaunchKernelGGL(hipns::k_replace, dim3(R), dim3(1024), lds, as_stream(stream), runs_dev, states_dev, *bufs, 1);
+  HIPNS_CHECK_LAUNCH("k_replace(init)");
+  return HIPNS_OK;
+}
+
+int hipns_ext_step(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
+                   const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
+                   const hipns_ext_state_t* ext, int32_t mode, int32_t step, void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  int maxD, maxN, maxS;
+  if ((rc = check_runs(runs_host, R, bufs, &maxD, &maxN, &maxS))) return rc;
+  if (!runs_dev || !states_dev || !launch || !ext) return fail(HIPNS_EINVAL, "null pointer");
+  if (R > 65535) return fail(HIPNS_ERANGE, "at most 65535 runs per launch");
+  if (launch->lanes_per_chain != 2 && launch->lanes_per_chain != 4)
+    return fail(HIPNS_EINVAL, "lanes_per_chain must be 2 or 4");
+  if (launch->waves_per_block < 1 || launch->waves_per_block > 4)
+    return fail(HIPNS_EINVAL, "waves_per_block must be 1..4");
+  if (launch->max_steps < 2) return fail(HIPNS_EINVAL, "max_steps must be >= 2");
+  if (mode < 0 || mode > 2) return fail(HIPNS_EINVAL, "mode must be 0 (slice), 1 (uniform) or 2 (initial)");
+  if (step < 0) return fail(HIPNS_EINVAL, "step < 0");
+  if (!ext->U0 || !ext->nv || !ext->prop_U || !ext->logL0 || !ext->prop_logL || !ext->left || !ext->right ||
+      !ext->t || !ext->s || !ext->k || !ext->evals || !ext->flags)
+    return fail(HIPNS_EINVAL, "null ext state array");
+  const int lpc = bufs->dps > 64 ? 4 : launch->lanes_per_chain;
+  const int maxC = mode == hipns::MODE_INIT ? maxN : maxS;
+  hipError_t e = hipns::launch_ext_step(runs_dev, states_dev, *bufs, *ext, mode, step, launch->max_steps, R, maxC, lpc,
+                                        launch->waves_per_block, maxD, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "ext step launch");
   return HIPNS_OK;
 }
 

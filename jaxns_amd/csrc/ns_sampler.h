@@ -87,6 +87,9 @@ inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc, int G 
   return (size_t)(3 + K) * dps + 2 * K;
 }
 
+hipError_t launch_ext_step(const hipns_run_desc_t* runs, hipns_run_state_t* states, const hipns_buffers_t& B,
+                           const hipns_ext_state_t& E, int mode, int w, int max_steps, int R, int max_chains, int lpc,
+                           int wpb, int dmax, hipStream_t s);
 hipError_t launch_linreg_stats(const float* U, int64_t B, double* logL, double* rss, double* yr,
                                const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int lpc, int dmax,
                                size_t lds, hipStream_t s, const LinRegLds& lr);

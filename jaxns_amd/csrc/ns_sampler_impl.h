@@ -524,6 +524,89 @@ __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, 
 }
 
 
+// ---------------------------------------------------------------------------
+// slice-sampler steps shared by the fused kernel and the external-likelihood steps
+// ---------------------------------------------------------------------------
+
+// get_seed_point (uni_slice_sampler.py:343-358): uniform choice among live points [first, N) above the
+// contour, the first i with (i - first + 1) >= M (1 - u), i.e. first + ceil(M (1 - u)) - 1
+// (random.py:57-60)
+__device__ __forceinline__ int seed_index_dev(int chain, uint32_t it32, uint32_t k0, uint32_t k1, int first, int N) {
+  u32x4 w = philox4x32((uint32_t)chain, it32, 0xFFFFFFFFu, RNG_SEED, k0, k1);
+  double u = u01d(w.x, w.y);
+  int M = N - first;
+  int idx = M > 0 ? first + (int)ceil((double)M * (1.0 - u)) - 1 : 0;
+  return idx < 0 ? 0 : (idx > N - 1 ? N - 1 : idx);
+}
+
+// _sample_direction (uni_slice_sampler.py:23-38): z ~ N(0, I), n = z / |z|.  One Philox block feeds
+// the 8 owned dimensions of a group through Box-Muller on 16-bit uniform halves.  Only the
+// direction matters (t scales with 1/|n|), so the hardware rsq normalises it.
+template <int LPC, int NJ>
+__device__ __forceinline__ void slice_direction(float (&nv)[NJ], int chain, uint32_t it32, int si, int D, int nj,
+                                                int h, uint32_t k0, uint32_t k1) {
+  if (D == 1) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) nv[j] = (owned_dim<LPC>(h, j) == 0) ? 1.0f : 0.0f;
+    return;
+  }
+  float ss = 0.0f;
+#pragma unroll
+  for (int j = 0; j < NJ; j += 8) {
+    if (j < nj) {
+      const int d0 = owned_dim<LPC>(h, j);
+      u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)si, RNG_DIR | (uint32_t)(d0 >> 3), k0, k1);
+      const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float u1 = (float)((ws[q] & 0xFFFFu) + 1u) * 1.52587890625e-5f;  // (0, 1]
+        float u2 = (float)(ws[q] >> 16) * 1.52587890625e-5f;             // [0, 1)
+        float rr = __builtin_amdgcn_sqrtf(-2.0f * __logf(u1));
+        float sn, cs;
+        __sincosf(6.28318530717958648f * u2, &sn, &cs);
+        float z0 = d0 + 2 * q < D ? rr * cs : 0.0f;
+        float z1 = d0 + 2 * q + 1 < D ? rr * sn : 0.0f;
+        nv[j + 2 * q] = z0;
+        nv[j + 2 * q + 1] = z1;
+        ss = fmaf(z0, z0, ss);
+        ss = fmaf(z1, z1, ss);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) nv[j + q] = 0.0f;
+    }
+  }
+  ss = chain_sum<LPC>(ss);
+  const float inv = __builtin_amdgcn_rsqf(ss);
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) nv[j] = nv[j] * inv;
+}
+
+// _slice_bounds (uni_slice_sampler.py:41-64).  Padding slots (beyond D or nj) hold U0 = 0.5 and n = +0,
+// whose quotients are +-inf and never the bound, so every slot is taken unconditionally.
+template <int LPC, int NJ>
+__device__ __forceinline__ void slice_bracket(const float (&U0)[NJ], const float (&nv)[NJ], float& left, float& right) {
+  float rmin = __builtin_inff(), lmax = -__builtin_inff();
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) bound_candidates(U0[j], nv[j], rmin, lmax);
+  right = chain_min<LPC>(rmin);
+  left = chain_max<LPC>(lmax);
+}
+
+// a uniform point of the unit cube for attempt `attempt` of a chain (Model.sample_U; padding 0.5)
+template <int LPC, int NJ>
+__device__ __forceinline__ void uniform_point(float (&up)[NJ], int chain, uint32_t it32, int attempt, int D, int nj,
+                                              int h, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int j = 0; j < NJ; j += 4) {
+    int d0 = owned_dim<LPC>(h, j);
+    u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)attempt, RNG_UNIF | (uint32_t)(d0 >> 2), k0, k1);
+    float uu[4] = {u01f(w.x), u01f(w.y), u01f(w.z), u01f(w.w)};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) up[j + q] = (j < nj && d0 + q < D) ? uu[q] : 0.5f;
+  }
+}
+
 template <int LPC, int NJ, int FAM, int MODE>
 __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   constexpr int CW = HIPNS_WAVE / LPC;
@@ -592,13 +675,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
       if (MODE == MODE_INJ) {
         idx = A.inj_seed[chain];
       } else {
-        u32x4 w = philox4x32((uint32_t)chain, it32, 0xFFFFFFFFu, RNG_SEED, k0, k1);
-        double u = u01d(w.x, w.y);
-        int a = st->first_seed;
-        int M = N - a;
-        // first i >= a with (i - a + 1) >= M (1 - u): i = a + ceil(M(1-u)) - 1 (random.py:57-60)
-        idx = M > 0 ? a + (int)ceil((double)M * (1.0 - u)) - 1 : 0;
-        idx = idx < 0 ? 0 : (idx > N - 1 ? N - 1 : idx);
+        idx = seed_index_dev(chain, it32, k0, k1, st->first_seed, N);
       }
     }
     const float* src = A.B.live_U[cur] + (run.live_off + idx) * (int64_t)dps;
@@ -632,64 +709,13 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
         const int d = owned_dim<LPC>(h, j);
         nv[j] = (j < nj && d < D) ? dsrc[d] : 0.0f;
       }
-    } else if (D == 1) {
-#pragma unroll
-      for (int j = 0; j < NJ; ++j) nv[j] = (owned_dim<LPC>(h, j) == 0) ? 1.0f : 0.0f;
     } else {
-      // _sample_direction (uni_slice_sampler.py:23-38): z ~ N(0, I), n = z / |z|.  One
-      // Philox block feeds the 8 owned dimensions of a group through Box-Muller on
-      // 16-bit uniform halves.
-      float ss = 0.0f;
-#pragma unroll
-      for (int j = 0; j < NJ; j += 8) {
-        if (j < nj) {
-          const int d0 = owned_dim<LPC>(h, j);
-          u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)si, RNG_DIR | (uint32_t)(d0 >> 3), k0, k1);
-          const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            float u1 = (float)((ws[q] & 0xFFFFu) + 1u) * 1.52587890625e-5f;  // (0, 1]
-            float u2 = (float)(ws[q] >> 16) * 1.52587890625e-5f;             // [0, 1)
-            float rr = __builtin_amdgcn_sqrtf(-2.0f * __logf(u1));
-            float sn, cs;
-            __sincosf(6.28318530717958648f * u2, &sn, &cs);
-            float z0 = d0 + 2 * q < D ? rr * cs : 0.0f;
-            float z1 = d0 + 2 * q + 1 < D ? rr * sn : 0.0f;
-            nv[j + 2 * q] = z0;
-            nv[j + 2 * q + 1] = z1;
-            ss = fmaf(z0, z0, ss);
-            ss = fmaf(z1, z1, ss);
-          }
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) nv[j + q] = 0.0f;
-        }
-      }
-      // only the direction matters (t scales with 1/|n|), so the hardware rsq suffices
-      ss = chain_sum<LPC>(ss);
-      const float inv = __builtin_amdgcn_rsqf(ss);
-#pragma unroll
-      for (int j = 0; j < NJ; ++j) nv[j] = nv[j] * inv;
+      slice_direction<LPC, NJ>(nv, chain, it32, si, D, nj, h, k0, k1);
     }
-    // _slice_bounds (uni_slice_sampler.py:41-64).  Padding slots (beyond D or nj) hold U0 = 0.5 and
-    // n = +0, whose quotients are +-inf and never the bound, so every slot is taken unconditionally.
-    float rmin = __builtin_inff(), lmax = -__builtin_inff();
-#pragma unroll
-    for (int j = 0; j < NJ; ++j) bound_candidates(U0[j], nv[j], rmin, lmax);
-    right = chain_min<LPC>(rmin);
-    left = chain_max<LPC>(lmax);
+    slice_bracket<LPC, NJ>(U0, nv, left, right);
   };
 
-  auto draw_uniform_point = [&](int attempt) {
-#pragma unroll
-    for (int j = 0; j < NJ; j += 4) {
-      int d0 = owned_dim<LPC>(h, j);
-      u32x4 w = philox4x32((uint32_t)chain, it32, (uint32_t)attempt, RNG_UNIF | (uint32_t)(d0 >> 2), k0, k1);
-      float uu[4] = {u01f(w.x), u01f(w.y), u01f(w.z), u01f(w.w)};
-#pragma unroll
-      for (int q = 0; q < 4; ++q) up[j + q] = (j < nj && d0 + q < D) ? uu[q] : 0.5f;
-    }
-  };
+  auto draw_uniform_point = [&](int attempt) { uniform_point<LPC, NJ>(up, chain, it32, attempt, D, nj, h, k0, k1); };
 
   bool need_begin = (MODE == MODE_SLICE || MODE == MODE_INJ) && alive;
   bool need_shrink = false;

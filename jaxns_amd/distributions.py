@@ -41,12 +41,14 @@ class Normal(Distribution):
     """tfpd.Normal(loc, scale): x = loc + scale * ndtri(u)."""
 
     def __init__(self, loc=0.0, scale=1.0):
+        scalar = np.ndim(loc) == 0 and np.ndim(scale) == 0
         loc = np.atleast_1d(np.asarray(loc, dtype=np.float64))
         scale = np.atleast_1d(np.asarray(scale, dtype=np.float64))
         n = max(loc.size, scale.size)
         self.loc = _vec(loc, n)
         self.scale = _vec(scale, n)
         self.num_dims = n
+        self.event_shape = () if scalar else (n,)
 
     def affine_form(self):
         return np.full(self.num_dims, Q_NDTRI, np.int32), self.loc, np.diag(self.scale)
@@ -63,12 +65,14 @@ class Uniform(Distribution):
     """tfpd.Uniform(low, high): x = low + (high - low) u."""
 
     def __init__(self, low=0.0, high=1.0):
+        scalar = np.ndim(low) == 0 and np.ndim(high) == 0
         low = np.atleast_1d(np.asarray(low, dtype=np.float64))
         high = np.atleast_1d(np.asarray(high, dtype=np.float64))
         n = max(low.size, high.size)
         self.low = _vec(low, n)
         self.high = _vec(high, n)
         self.num_dims = n
+        self.event_shape = () if scalar else (n,)
 
     def affine_form(self):
         return np.full(self.num_dims, Q_IDENTITY, np.int32), self.low, np.diag(self.high - self.low)
@@ -88,6 +92,7 @@ class MultivariateNormalTriL(Distribution):
         self.loc = np.atleast_1d(np.asarray(loc, dtype=np.float64))
         self.scale_tril = np.tril(np.asarray(scale_tril, dtype=np.float64))
         self.num_dims = self.loc.size
+        self.event_shape = (self.num_dims,)
         if self.scale_tril.shape != (self.num_dims, self.num_dims):
             raise ValueError("scale_tril must be [D, D]")
 

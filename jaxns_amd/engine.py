@@ -105,10 +105,18 @@ class DeviceBatch:
         self.R = len(specs)
         fam = specs[0].model.log_likelihood
         self.family = getattr(fam, "family", None)
-        if self.family is None:
-            raise NotImplementedError("the device engine needs a fused likelihood family (jaxns_amd.likelihoods)")
+        self.external = self.family is None
+        if self.external:
+            # a torch callable: evaluated between one-step sampler launches (hipns_ext_step)
+            if not callable(fam):
+                raise TypeError("log_likelihood must be a likelihood family or a callable")
+            if any(s.model is not specs[0].model for s in specs):
+                raise ValueError("runs with a callable likelihood must share one Model")
+            from .likelihoods import TorchLikelihood
+            self.ext_like = TorchLikelihood(fam, specs[0].model)
+            self.family = _lib.FAM_MVN  # launch-struct placeholder; no fused family is launched
         for s in specs:
-            if getattr(s.model.log_likelihood, "family", None) != self.family:
+            if not self.external and getattr(s.model.log_likelihood, "family", None) != self.family:
                 raise ValueError("all runs of a batch must share the likelihood family")
         self.K = getattr(fam, "K", 1) if self.family == _lib.FAM_GMM else 1
         if self.family == _lib.FAM_GMM and any(getattr(s.model.log_likelihood, "K", 0) != self.K for s in specs):
@@ -126,7 +134,10 @@ class DeviceBatch:
         # parameters
         prms, consts = [], []
         for s in specs:
-            p, c = s.model.log_likelihood.device_params(s.model, dps)
+            if self.external:
+                p, c = np.zeros(4, np.float32), 0.0
+            else:
+                p, c = s.model.log_likelihood.device_params(s.model, dps)
             prms.append(p)
             consts.append(c)
         poffs = np.cumsum([0] + [p.size for p in prms])
@@ -196,6 +207,24 @@ class DeviceBatch:
             raise ValueError("lr_terms must be 1 (plain FP32), 3 (BF16x3) or 4")
         self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, lr_ndata, max_steps, lr_panels,
                              lr_terms, lr_off)
+        if self.external:
+            i32 = torch.int32
+            self.ext_U0 = torch.full((sumN, dps), 0.5, dtype=f32, device=dev)
+            self.ext_nv = torch.zeros((sumN, dps), dtype=f32, device=dev)
+            self.ext_prop_U = torch.full((sumN, dps), 0.5, dtype=f32, device=dev)
+            self.ext_logL0 = torch.zeros(sumN, dtype=f64, device=dev)
+            self.ext_prop_logL = torch.zeros(sumN, dtype=f64, device=dev)
+            self.ext_left = torch.zeros(sumN, dtype=f32, device=dev)
+            self.ext_right = torch.zeros(sumN, dtype=f32, device=dev)
+            self.ext_t = torch.zeros(sumN, dtype=f32, device=dev)
+            self.ext_s = torch.zeros(sumN, dtype=i32, device=dev)
+            self.ext_k = torch.zeros(sumN, dtype=i32, device=dev)
+            self.ext_evals = torch.zeros(sumN, dtype=i32, device=dev)
+            self.ext_flags = torch.zeros(sumN, dtype=i32, device=dev)
+            e = _lib.ExtState()
+            for name in ("U0", "nv", "prop_U", "logL0", "prop_logL", "left", "right", "t", "s", "k", "evals", "flags"):
+                setattr(e, name, ptr(getattr(self, "ext_" + name)))
+            self.ext = e
         self._graph = None
         self._ev_ws = None
         self._graph_iters = 0
@@ -206,9 +235,32 @@ class DeviceBatch:
         return _lib.stream_ptr(stream)
 
     def init_state(self, stream=None):
+        if self.external:
+            s = self._s(stream)
+            check(lib().hipns_init_prepare(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
+                                           C.byref(self.bufs), s), "hipns_init_prepare")
+            self._ext_steps(2, stream)
+            check(lib().hipns_init_finalize(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
+                                            C.byref(self.bufs), s), "hipns_init_finalize")
+            return
         check(lib().hipns_create_init_state(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
                                             C.byref(self.bufs), C.byref(self.launch), None, 0, self._s(stream)),
               "hipns_create_init_state")
+
+    def _ext_steps(self, mode: int, stream=None):
+        """Step every chain until none runs, evaluating each step's proposals with the callable
+        (Model.forward over the batch)."""
+        s = self._s(stream)
+        self.ext_flags.zero_()
+        w = 0
+        while True:
+            check(lib().hipns_ext_step(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
+                                       C.byref(self.bufs), C.byref(self.launch), C.byref(self.ext), mode, w, s),
+                  "hipns_ext_step")
+            if not bool((self.ext_flags & 1).any()):
+                break
+            self.ext_prop_logL.copy_(self.ext_like(self.ext_prop_U))
+            w += 1
 
     def check_termination(self, term: dict, set_no_seed: bool = True, stream=None):
         tc = _lib.make_term_cond(**term)
@@ -218,6 +270,14 @@ class DeviceBatch:
 
     def collect_shell(self, term: dict, stream=None):
         tc = _lib.make_term_cond(**term)
+        if self.external:
+            s = self._s(stream)
+            args = (ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host, C.byref(self.bufs))
+            check(lib().hipns_shell_discard(*args, s), "hipns_shell_discard")
+            self._ext_steps(1 if self.runs_host[0].sampler == 1 else 0, stream)
+            check(lib().hipns_shell_replace(*args, s), "hipns_shell_replace")
+            check(lib().hipns_shell_register(*args, C.byref(tc), s), "hipns_shell_register")
+            return
         check(lib().hipns_collect_shell(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
                                         C.byref(self.bufs), C.byref(self.launch), C.byref(tc), None, 0,
                                         self._s(stream)), "hipns_collect_shell")
@@ -267,7 +327,7 @@ class DeviceBatch:
     def _loop(self, term, *, graph_iters, max_iterations, stream):
         torch = _torch()
         it = 0
-        if graph_iters and graph_iters > 0:
+        if graph_iters and graph_iters > 0 and not self.external:
             g = self._get_graph(term, graph_iters)
             while not np.all(self.done_flags()) and it < max_iterations:
                 g.replay()

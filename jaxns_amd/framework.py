@@ -117,6 +117,36 @@ class Model:
         except Exception:
             got = None
         self._output_is_concat = got is not None and got.shape == flat.shape and np.allclose(got, flat)
+        # the likelihood's arguments: the generator's return value, a tuple of yielded values or their
+        # concatenation (checked by running the generator with distinct U values)
+        U = np.linspace(0.1, 0.9, max(self.U_ndims, 1))
+        _, out2, xs2 = _run_generator(self.prior_model, U)
+        outs2 = out2 if isinstance(out2, tuple) else (out2,)
+        self._arg_index = None
+        idx = []
+        for o in outs2:
+            hit = [i for i, x in enumerate(xs2) if np.shape(x) == np.shape(o) and np.allclose(np.asarray(x), o)]
+            if not hit:
+                idx = None
+                break
+            idx.append(hit[0])
+        if idx is not None and isinstance(out2, tuple):
+            self._arg_index = idx
+
+    def likelihood_args_torch(self, X):
+        """The likelihood's arguments for a batch X [B, U_ndims] of prior outputs (torch): the prior
+        values the generator returns, each [B, ...], or X itself when it returns their concatenation."""
+        if self._arg_index is None:
+            if not self._output_is_concat:
+                raise NotImplementedError("the prior model must return its prior values or their concatenation")
+            return (X,)
+        offs = np.cumsum([0] + [p.base_ndims for p in self.priors])
+        args = []
+        for i in self._arg_index:
+            p = self.priors[i]
+            x = X[:, offs[i]:offs[i + 1]]
+            args.append(x.reshape((X.shape[0],) + tuple(getattr(p.dist, "event_shape", (p.base_ndims,)))))
+        return tuple(args)
 
     # ------------------------------------------------------------------ program
     def prior_program(self):
