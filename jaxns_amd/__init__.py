@@ -16,9 +16,9 @@ from .likelihoods import (GaussianMixtureLikelihood, LinearRegressionLikelihood,
 from .nested_sampler import DefaultNestedSampler, NestedSampler, NestedSamplerState
 from .types import (NestedSamplerResults, TerminationCondition, TerminationConditionConjunction,
                     TerminationConditionDisjunction)
-from .utils import load_results, save_results, summary
+from .utils import load_results, sample_evidence, save_results, summary
 
 __all__ = ["Model", "Prior", "NestedSampler", "DefaultNestedSampler", "NestedSamplerState", "TerminationCondition",
            "TerminationConditionConjunction", "TerminationConditionDisjunction", "NestedSamplerResults",
            "MultivariateNormalLikelihood", "GaussianMixtureLikelihood", "LinearRegressionLikelihood", "RegressionData", "summary",
-           "save_results", "load_results", "distributions", "InvalidPriorName"]
+           "save_results", "load_results", "sample_evidence", "distributions", "InvalidPriorName"]

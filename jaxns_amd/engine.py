@@ -118,6 +118,9 @@ class DeviceBatch:
         for s in specs:
             if not self.external and getattr(s.model.log_likelihood, "family", None) != self.family:
                 raise ValueError("all runs of a batch must share the likelihood family")
+            if not self.external and s.model.dynamic_prior:
+                raise ValueError("a prior that depends on other prior values needs a callable likelihood: the "
+                                 "fused kernels evaluate a fixed affine prior")
         self.K = getattr(fam, "K", 1) if self.family == _lib.FAM_GMM else 1
         if self.family == _lib.FAM_GMM and any(getattr(s.model.log_likelihood, "K", 0) != self.K for s in specs):
             raise ValueError("all mixture runs of a batch must have the same number of components")

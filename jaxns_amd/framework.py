@@ -117,36 +117,45 @@ class Model:
         except Exception:
             got = None
         self._output_is_concat = got is not None and got.shape == flat.shape and np.allclose(got, flat)
-        # the likelihood's arguments: the generator's return value, a tuple of yielded values or their
-        # concatenation (checked by running the generator with distinct U values)
-        U = np.linspace(0.1, 0.9, max(self.U_ndims, 1))
-        _, out2, xs2 = _run_generator(self.prior_model, U)
-        outs2 = out2 if isinstance(out2, tuple) else (out2,)
-        self._arg_index = None
-        idx = []
-        for o in outs2:
-            hit = [i for i, x in enumerate(xs2) if np.shape(x) == np.shape(o) and np.allclose(np.asarray(x), o)]
-            if not hit:
-                idx = None
-                break
-            idx.append(hit[0])
-        if idx is not None and isinstance(out2, tuple):
-            self._arg_index = idx
+        # a prior whose parameters depend on earlier prior values (e.g. Normal(loc, scale=x)) has no
+        # fixed affine program: detect it by running the generator at two other points
+        self.dynamic_prior = False
+        forms = []
+        for v in (0.3, 0.7):
+            pr, _, _ = _run_generator(self.prior_model, np.full(self.U_ndims, v))
+            forms.append([p.dist.affine_form() for p in pr if p._type == "dist"])
+        for fa, fb in zip(*forms):
+            if any(not np.allclose(np.asarray(x), np.asarray(y)) for x, y in zip(fa, fb)):
+                self.dynamic_prior = True
 
-    def likelihood_args_torch(self, X):
-        """The likelihood's arguments for a batch X [B, U_ndims] of prior outputs (torch): the prior
-        values the generator returns, each [B, ...], or X itself when it returns their concatenation."""
-        if self._arg_index is None:
-            if not self._output_is_concat:
-                raise NotImplementedError("the prior model must return its prior values or their concatenation")
-            return (X,)
-        offs = np.cumsum([0] + [p.base_ndims for p in self.priors])
-        args = []
-        for i in self._arg_index:
-            p = self.priors[i]
-            x = X[:, offs[i]:offs[i + 1]]
-            args.append(x.reshape((X.shape[0],) + tuple(getattr(p.dist, "event_shape", (p.base_ndims,)))))
-        return tuple(args)
+    def likelihood_inputs_torch(self, U):
+        """The prior model's return value for a torch batch U [B, >= U_ndims] in float64, computed by
+        driving the generator with batched prior values (framework/ops.py:240-267, prepare_input):
+        each yielded prior maps its U slice through its quantile and bijector with the parameters it
+        was given, which may themselves be earlier prior values.  Returned as a tuple of likelihood
+        arguments (a tuple return value is unpacked, as in jaxns)."""
+        import torch
+        Bn = U.shape[0]
+        u64 = U[:, :self.U_ndims].double()
+        gen = self.prior_model()
+        resp = None
+        i = o = 0
+        while True:
+            try:
+                p = gen.send(resp)
+            except StopIteration as e:
+                out = e.value
+                break
+            if p._type == "value":
+                resp = torch.as_tensor(p.value, dtype=torch.float64, device=U.device)
+                continue
+            ref = self.priors[i]
+            w = ref.base_ndims
+            shp = tuple(getattr(ref.dist, "event_shape", (w,)))
+            resp = p.dist.forward_torch(u64[:, o:o + w].reshape((Bn,) + shp))
+            i += 1
+            o += w
+        return out if isinstance(out, tuple) else (out,)
 
     # ------------------------------------------------------------------ program
     def prior_program(self):

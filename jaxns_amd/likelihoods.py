@@ -244,9 +244,10 @@ class LinearRegressionLikelihood(LikelihoodFamily):
 
 
 class TorchLikelihood:
-    """A likelihood given as a batched torch callable, evaluated between sampler steps (the slow
-    general form).  ``fn`` receives the prior values as torch float64 tensors on the device, batched
-    along a leading axis, and returns log-likelihoods of shape [B]; NaN is read as -inf."""
+    """A likelihood given as a batched torch callable, evaluated between sampler steps (the general
+    form).  ``fn`` receives what the prior model returns (a tuple is unpacked into arguments), as
+    float64 device tensors batched along a leading axis, and returns log-likelihoods of shape [B]
+    (a scalar is broadcast); NaN is read as -inf."""
 
     family = None
 
@@ -254,38 +255,11 @@ class TorchLikelihood:
         import torch
         self.fn = fn
         self.model = model
-        qkind, loc, S = model.prior_program()
-        self.D = model.U_ndims
-        self._qkind = qkind
-        self._loc = loc
-        self._S = S
-        self._dev = None
         self.torch = torch
-
-    def _consts(self, device):
-        if self._dev != device:
-            t = self.torch
-            self._qk = t.from_numpy(self._qkind.astype(np.bool_)).to(device)
-            self._loc_t = t.from_numpy(self._loc).to(device)
-            self._St = t.from_numpy(np.ascontiguousarray(self._S.T)).to(device)
-            self._diag = bool(np.allclose(self._S, np.diag(np.diag(self._S))))
-            self._sd = t.from_numpy(np.diag(self._S).copy()).to(device)
-            self._dev = device
-        return self
-
-    def transform(self, U):
-        """X = loc + S q(U) in float64 (framework/wrapped_tfp_distribution.py:77-84)."""
-        t = self.torch
-        self._consts(U.device)
-        u = U[:, :self.D].double()
-        q = t.where(self._qk[None, :], t.special.ndtri(u), u)
-        if self._diag:
-            return self._loc_t[None, :] + q * self._sd[None, :]
-        return self._loc_t[None, :] + q @ self._St
 
     def __call__(self, U):
         t = self.torch
-        X = self.transform(U)
-        L = self.fn(*self.model.likelihood_args_torch(X))
-        L = t.as_tensor(L, device=U.device).reshape(-1).to(t.float64)
+        L = self.fn(*self.model.likelihood_inputs_torch(U))
+        L = t.as_tensor(L, dtype=t.float64, device=U.device)
+        L = L.expand(U.shape[0]) if L.dim() == 0 else L.reshape(-1)
         return t.where(t.isnan(L), t.full_like(L, -float("inf")), L)

@@ -99,3 +99,21 @@ def summary(results: NestedSamplerResults, with_parametrised: bool = False, f_ob
         else:
             raise TypeError(f"Invalid f_obj: {type(f_obj)}")
     return out
+
+
+def sample_evidence(key, num_live_points_per_sample, log_L_samples, S: int = 100) -> np.ndarray:
+    """Samples of log Z by simulating the shrinkage, T_i ~ Beta(n_i, 1), X_i = X_{i-1} T_i,
+    Z = sum (X_{i-1} - X_i) L_i (utils.py:433-480).  ``key`` is an integer seed."""
+    n = np.asarray(num_live_points_per_sample, dtype=np.float64)
+    L = np.asarray(log_L_samples, dtype=np.float64)
+    rng = np.random.default_rng(int(np.asarray(key).ravel()[-1]))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        log_T = np.log(rng.uniform(size=(S, n.size))) / n[None, :]
+        log_X = np.cumsum(log_T, axis=1)
+        log_X_prev = np.concatenate([np.zeros((S, 1)), log_X[:, :-1]], axis=1)
+        # log(X_prev - X) = log X_prev + log(1 - T)
+        log_dX = log_X_prev + np.log1p(-np.exp(log_T))
+        terms = log_dX + L[None, :]
+        m = np.max(terms, axis=1, keepdims=True)
+        m = np.where(np.isfinite(m), m, 0.0)
+        return (m[:, 0] + np.log(np.sum(np.exp(terms - m), axis=1)))

@@ -220,3 +220,18 @@ def test_results_json_roundtrip(tmp_path):
             np.testing.assert_array_equal(a, b)
     out = J.summary(back)
     assert "logZ=-1.50 +- 0.10" in out
+
+
+def test_sample_evidence_moments():
+    """sample_evidence (utils.py:433-480) draws T_i ~ Beta(n_i, 1) and sums Z = sum_i X_{i-1}(1 - T_i) L_i
+    (a right-point rule, unlike the trapezoid of compute_evidence_stats).  Its mean has the closed form
+    E[Z] = sum_i L_i / (n_i + 1) prod_{j<i} n_j / (n_j + 1)."""
+    rng = np.random.default_rng(3)
+    M = 400
+    logL = np.sort(rng.normal(size=M)) * 3
+    n = np.full(M, 20.0)
+    n[-20:] = np.arange(20, 0, -1)
+    EX_prev = np.concatenate([[1.0], np.cumprod(n / (n + 1.0))[:-1]])
+    EZ = np.sum(np.exp(logL) * EX_prev / (n + 1.0))
+    Z = np.exp(J.sample_evidence(5, n, logL, S=20000))
+    assert abs(Z.mean() - EZ) < 4 * Z.std() / np.sqrt(Z.size)
