@@ -259,7 +259,9 @@ typedef struct hipns_ext_state {
   int32_t* s;         /* [C] slice index */
   int32_t* k;         /* [C] shrinks of the current slice */
   int32_t* evals;     /* [C] likelihood evaluations of the chain */
-  int32_t* flags;     /* [C] bit 0 running, bit 1 begins a slice, bit 2 shrinks */
+  int32_t* flags;     /* [C] bit 0 running, bit 1 begins a slice, bit 2 shrinks, bit 3 asks for the
+                       * gradient of logL at the prop_U row (mode 4) */
+  float* grad;        /* [C][dps] written by the caller for rows with bit 3: d logL / dU (mode 4 only) */
 } hipns_ext_state_t;
 
 /* One step of every chain of the batch (UniDimSliceSampler.get_sample_from_seed,
@@ -267,7 +269,10 @@ typedef struct hipns_ext_state {
  * uniform draws, initialisation.py:20-84): step 0 seeds the chains and writes their first
  * proposals; step w > 0 first consumes prop_logL of step w-1 (acceptance, shrink, phantoms,
  * outputs into new_U/new_logL/new_evals), then writes the proposals of step w.  mode: 0 slice
- * shell, 1 uniform-sampler shell, 2 initial live points.  A shell is hipns_shell_discard, steps
+ * shell, 1 uniform-sampler shell, 2 initial live points, 4 gradient-guided slice shell
+ * (uni_slice_sampler.py:255-271: after each accepted slice the chain asks for d logL / dU at the
+ * accepted point, counted as one likelihood evaluation, and reflects its direction about it;
+ * a gradient of norm < 1e-10 or non-finite gives a random direction).  A shell is hipns_shell_discard, steps
  * until no chain runs (flags bit 0), hipns_shell_replace and hipns_shell_register; the initial
  * state is hipns_init_prepare, mode-2 steps, hipns_init_finalize. */
 int hipns_ext_step(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,

@@ -71,7 +71,7 @@ class Launch(C.Structure):
 
 class ExtState(C.Structure):
     _fields_ = [("U0", _vp), ("nv", _vp), ("prop_U", _vp), ("logL0", _vp), ("prop_logL", _vp), ("left", _vp),
-                ("right", _vp), ("t", _vp), ("s", _vp), ("k", _vp), ("evals", _vp), ("flags", _vp)]
+                ("right", _vp), ("t", _vp), ("s", _vp), ("k", _vp), ("evals", _vp), ("flags", _vp), ("grad", _vp)]
 
 
 assert C.sizeof(RunDesc) == 88, C.sizeof(RunDesc)

@@ -289,7 +289,9 @@ int hipns_ext_step(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_d
   if (launch->waves_per_block < 1 || launch->waves_per_block > 4)
     return fail(HIPNS_EINVAL, "waves_per_block must be 1..4");
   if (launch->max_steps < 2) return fail(HIPNS_EINVAL, "max_steps must be >= 2");
-  if (mode < 0 || mode > 2) return fail(HIPNS_EINVAL, "mode must be 0 (slice), 1 (uniform) or 2 (initial)");
+  if (mode < 0 || mode > 4 || mode == 3)
+    return fail(HIPNS_EINVAL, "mode must be 0 (slice), 1 (uniform), 2 (initial) or 4 (gradient-guided slice)");
+  if (mode == hipns::MODE_GSLICE && !ext->grad) return fail(HIPNS_EINVAL, "mode 4 needs ext->grad");
   if (step < 0) return fail(HIPNS_EINVAL, "step < 0");
   if (!ext->U0 || !ext->nv || !ext->prop_U || !ext->logL0 || !ext->prop_logL || !ext->left || !ext->right ||
       !ext->t || !ext->s || !ext->k || !ext->evals || !ext->flags)

@@ -22,6 +22,8 @@ struct ExtArgs {
 template <int LPC, int NJ, int MODE>
 __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
   constexpr int CW = HIPNS_WAVE / LPC;
+  constexpr bool SL = (MODE == MODE_SLICE || MODE == MODE_GSLICE);  // slice chains
+  constexpr bool GG = (MODE == MODE_GSLICE);                        // gradient-guided directions
   const int r = blockIdx.y;
   const hipns_run_desc_t run = A.runs[r];
   hipns_run_state_t* st = A.states + r;
@@ -53,7 +55,7 @@ __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
   double logL0;
   float left, right, t;
   int s, k, evals;
-  bool alive, need_begin, need_shrink;
+  bool alive, need_begin, need_shrink, need_grad, dir_ready;
 
   if (w == 0) {
     alive = valid;
@@ -61,16 +63,16 @@ __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
     k = 0;
     evals = 0;
     left = right = t = 0.0f;
-    need_begin = (MODE == MODE_SLICE) && alive;
-    need_shrink = false;
-    logL0 = (MODE == MODE_SLICE) ? 0.0 : -__builtin_inf();
+    need_begin = SL && alive;
+    need_shrink = need_grad = dir_ready = false;
+    logL0 = SL ? 0.0 : -__builtin_inf();
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       U0[j] = 0.5f;
       nv[j] = 0.0f;
       up[j] = 0.5f;
     }
-    if (MODE == MODE_SLICE && alive) {
+    if (SL && alive) {
       const int cur = st->buf;
       const int idx = seed_index_dev(chain, it32, k0, k1, st->first_seed, N);
       const float* src = A.B.live_U[cur] + (run.live_off + idx) * (int64_t)dps;
@@ -83,6 +85,8 @@ __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
     alive = (fl & 1) != 0;
     need_begin = (fl & 2) != 0;
     need_shrink = (fl & 4) != 0;
+    need_grad = (fl & 8) != 0;
+    dir_ready = (fl & 16) != 0;
     logL0 = E.logL0[o];
     left = E.left[o];
     right = E.right[o];
@@ -100,7 +104,56 @@ __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
       nv[j] = (j < nj) ? nvp[d] : 0.0f;
       up[j] = (j < nj) ? upp[d] : 0.5f;
     }
-    if (alive) {
+    if (alive && GG && need_grad) {
+      // the gradient at the accepted point (one likelihood evaluation), then the Householder
+      // reflection of the slice's direction about it (uni_slice_sampler.py:255-271)
+      ++evals;
+      need_grad = false;
+      if (s == num_slices) {
+        alive = false;
+        float* dst = A.B.new_U + o * dps;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+          if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+        if (h == 0) {
+          A.B.new_logL[o] = logL0;
+          A.B.new_evals[o] = evals;
+        }
+      } else {
+        const float* gp = E.grad + o * dps;
+        float g[NJ];
+        float gg2 = 0.0f;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int d = owned_dim<LPC>(h, j);
+          g[j] = (j < nj && d < D) ? gp[d] : 0.0f;
+          gg2 = fmaf(g[j], g[j], gg2);
+        }
+        const float gn = __builtin_sqrtf(chain_sum<LPC>(gg2));
+        if (!(gn >= 1e-10f) || !__builtin_isfinite(gn)) {
+          slice_direction<LPC, NJ>(nv, chain, it32, s, D, nj, h, k0, k1);
+        } else {
+          float dot = 0.0f;
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) {
+            g[j] = g[j] / gn;
+            dot = fmaf(nv[j], g[j], dot);
+          }
+          dot = chain_sum<LPC>(dot);
+          float rn2 = 0.0f;
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) {
+            nv[j] = nv[j] - 2.0f * dot * g[j];
+            rn2 = fmaf(nv[j], nv[j], rn2);
+          }
+          const float inv = 1.0f / __builtin_sqrtf(chain_sum<LPC>(rn2));
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) nv[j] = nv[j] * inv;
+        }
+        need_begin = true;
+        dir_ready = true;
+      }
+    } else if (alive) {
       // the caller's Model.forward of the previous proposal; NaN -> -inf (framework/ops.py:302-326)
       double L = E.prop_logL[o];
       if (__builtin_isnan(L)) L = -__builtin_inf();
@@ -158,7 +211,9 @@ __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
             }
           }
           ++s;
-          if (s == num_slices) {
+          if (GG) {
+            need_grad = true;  // every accepted slice asks for the gradient, the last one included
+          } else if (s == num_slices) {
             alive = false;
             float* dst = A.B.new_U + o * dps;
 #pragma unroll
@@ -178,12 +233,16 @@ __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
     }
   }
 
-  // ---- the proposal of step w
-  if (alive) {
-    if (MODE == MODE_SLICE) {
+  // ---- the proposal of step w (or the point whose gradient is asked for)
+  if (alive && GG && need_grad) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) up[j] = U0[j];
+  } else if (alive) {
+    if (SL) {
       if (need_begin) {
         k = 0;
-        slice_direction<LPC, NJ>(nv, chain, it32, s, D, nj, h, k0, k1);
+        if (!dir_ready) slice_direction<LPC, NJ>(nv, chain, it32, s, D, nj, h, k0, k1);
+        dir_ready = false;
         slice_bracket<LPC, NJ>(U0, nv, left, right);
         need_begin = false;
       } else if (need_shrink) {
@@ -218,7 +277,8 @@ __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
     }
   }
   if (h == 0) {
-    E.flags[o] = (alive ? 1 : 0) | (need_begin ? 2 : 0) | (need_shrink ? 4 : 0);
+    E.flags[o] = (alive ? 1 : 0) | (need_begin ? 2 : 0) | (need_shrink ? 4 : 0) | (need_grad ? 8 : 0) |
+                 (dir_ready ? 16 : 0);
     E.logL0[o] = logL0;
     E.left[o] = left;
     E.right[o] = right;
@@ -264,6 +324,7 @@ hipError_t launch_ext_step(const hipns_run_desc_t* runs, hipns_run_state_t* stat
   A.w = w;
   if (mode == MODE_INIT) return launch_ext_mode<MODE_INIT>(A, R, max_chains, lpc, wpb, dmax, s);
   if (mode == MODE_UNIF) return launch_ext_mode<MODE_UNIF>(A, R, max_chains, lpc, wpb, dmax, s);
+  if (mode == MODE_GSLICE) return launch_ext_mode<MODE_GSLICE>(A, R, max_chains, lpc, wpb, dmax, s);
   return launch_ext_mode<MODE_SLICE>(A, R, max_chains, lpc, wpb, dmax, s);
 }
 

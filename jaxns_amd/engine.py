@@ -97,7 +97,7 @@ class DeviceBatch:
     """R independent runs sharing one likelihood family, advanced together."""
 
     def __init__(self, specs: List[RunSpec], *, lanes_per_chain: int = 2, waves_per_block: int = 4,
-                 max_steps: int = 512, device: str = "cuda", lr_terms: int = 3):
+                 max_steps: int = 512, device: str = "cuda", lr_terms: int = 3, gradient_guided: bool = False):
         torch = _torch()
         if not specs:
             raise ValueError("no runs")
@@ -105,15 +105,19 @@ class DeviceBatch:
         self.R = len(specs)
         fam = specs[0].model.log_likelihood
         self.family = getattr(fam, "family", None)
-        self.external = self.family is None
+        self.gradient_guided = bool(gradient_guided)
+        # gradient-guided slicing needs d logL / dU at accepted points, which the stepwise path
+        # takes from torch autograd; a fused family then runs through its torch form
+        self.external = self.family is None or self.gradient_guided
         if self.external:
             # a torch callable: evaluated between one-step sampler launches (hipns_ext_step)
-            if not callable(fam):
+            fn = fam.torch_log_prob if self.family is not None else fam
+            if not callable(fn):
                 raise TypeError("log_likelihood must be a likelihood family or a callable")
             if any(s.model is not specs[0].model for s in specs):
-                raise ValueError("runs with a callable likelihood must share one Model")
+                raise ValueError("runs of the stepwise path must share one Model")
             from .likelihoods import TorchLikelihood
-            self.ext_like = TorchLikelihood(fam, specs[0].model)
+            self.ext_like = TorchLikelihood(fn, specs[0].model)
             self.family = _lib.FAM_MVN  # launch-struct placeholder; no fused family is launched
         for s in specs:
             if not self.external and getattr(s.model.log_likelihood, "family", None) != self.family:
@@ -224,8 +228,10 @@ class DeviceBatch:
             self.ext_k = torch.zeros(sumN, dtype=i32, device=dev)
             self.ext_evals = torch.zeros(sumN, dtype=i32, device=dev)
             self.ext_flags = torch.zeros(sumN, dtype=i32, device=dev)
+            self.ext_grad = torch.zeros((sumN, dps), dtype=f32, device=dev)
             e = _lib.ExtState()
-            for name in ("U0", "nv", "prop_U", "logL0", "prop_logL", "left", "right", "t", "s", "k", "evals", "flags"):
+            for name in ("U0", "nv", "prop_U", "logL0", "prop_logL", "left", "right", "t", "s", "k", "evals", "flags",
+                         "grad"):
                 setattr(e, name, ptr(getattr(self, "ext_" + name)))
             self.ext = e
         self._graph = None
@@ -262,7 +268,16 @@ class DeviceBatch:
                   "hipns_ext_step")
             if not bool((self.ext_flags & 1).any()):
                 break
-            self.ext_prop_logL.copy_(self.ext_like(self.ext_prop_U))
+            if mode == 4 and bool((self.ext_flags & 8).any()):
+                torch = _torch()
+                with torch.enable_grad():
+                    U = self.ext_prop_U.detach().clone().requires_grad_(True)
+                    L = self.ext_like(U)
+                    g, = torch.autograd.grad(torch.where(torch.isfinite(L), L, torch.zeros_like(L)).sum(), U)
+                self.ext_prop_logL.copy_(L.detach())
+                self.ext_grad.copy_(torch.nan_to_num(g, nan=0.0, posinf=0.0, neginf=0.0))
+            else:
+                self.ext_prop_logL.copy_(self.ext_like(self.ext_prop_U))
             w += 1
 
     def check_termination(self, term: dict, set_no_seed: bool = True, stream=None):
@@ -277,7 +292,8 @@ class DeviceBatch:
             s = self._s(stream)
             args = (ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host, C.byref(self.bufs))
             check(lib().hipns_shell_discard(*args, s), "hipns_shell_discard")
-            self._ext_steps(1 if self.runs_host[0].sampler == 1 else 0, stream)
+            mode = 1 if self.runs_host[0].sampler == 1 else (4 if self.gradient_guided else 0)
+            self._ext_steps(mode, stream)
             check(lib().hipns_shell_replace(*args, s), "hipns_shell_replace")
             check(lib().hipns_shell_register(*args, C.byref(tc), s), "hipns_shell_register")
             return

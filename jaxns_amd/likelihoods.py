@@ -53,6 +53,14 @@ class MultivariateNormalLikelihood(LikelihoodFamily):
         Y = solve_triangular(self.scale_tril, (X - self.mean).T, lower=True)
         return self.log_const - 0.5 * np.sum(Y * Y, axis=0)
 
+    def torch_log_prob(self, x):
+        """The same density in torch (float64, batched [B, D]) for the stepwise path."""
+        import torch
+        L = torch.as_tensor(self.scale_tril, dtype=torch.float64, device=x.device)
+        mu = torch.as_tensor(self.mean, dtype=torch.float64, device=x.device)
+        z = torch.linalg.solve_triangular(L, (x - mu).T, upper=False)
+        return self.log_const - 0.5 * (z * z).sum(0)
+
     def device_params(self, model, dps):
         from scipy.linalg import solve_triangular
         D = model.U_ndims
@@ -89,6 +97,15 @@ class GaussianMixtureLikelihood(LikelihoodFamily):
         X = np.atleast_2d(np.asarray(X, dtype=np.float64))
         d2 = np.sum((X[:, None, :] - self.means[None]) ** 2, axis=-1)
         return logsumexp(self.logc[None] - 0.5 * d2 / self.sigmas[None] ** 2, axis=1)
+
+    def torch_log_prob(self, x):
+        """The same density in torch (float64, batched [B, D]) for the stepwise path."""
+        import torch
+        m = torch.as_tensor(self.means, dtype=torch.float64, device=x.device)
+        sg = torch.as_tensor(self.sigmas, dtype=torch.float64, device=x.device)
+        lc = torch.as_tensor(self.logc, dtype=torch.float64, device=x.device)
+        d2 = ((x[:, None, :] - m[None]) ** 2).sum(-1)
+        return torch.logsumexp(lc[None] - 0.5 * d2 / sg[None] ** 2, dim=1)
 
     def device_params(self, model, dps):
         D = model.U_ndims
@@ -202,6 +219,14 @@ class LinearRegressionLikelihood(LikelihoodFamily):
         W = np.atleast_2d(np.asarray(W, dtype=np.float64))
         R = self.y[None, :] - W @ self.X.T
         return self.log_const - 0.5 * np.sum(R * R, axis=1) / self.sigma ** 2
+
+    def torch_log_prob(self, w):
+        """The same density in torch (float64, batched [B, D]) for the stepwise path."""
+        import torch
+        X = torch.as_tensor(self.X, dtype=torch.float64, device=w.device)
+        y = torch.as_tensor(self.y, dtype=torch.float64, device=w.device)
+        R = y[None, :] - w @ X.T
+        return self.log_const - 0.5 * (R * R).sum(1) / self.sigma ** 2
 
     def device_params(self, model, dps):
         D = model.U_ndims

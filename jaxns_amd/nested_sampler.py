@@ -76,8 +76,7 @@ class NestedSampler:
         if max_samples is None:
             max_samples = self.c * (self.k + 1) * 100
         self.max_samples = int(max_samples)
-        if gradient_guided:
-            raise NotImplementedError("gradient_guided slice sampling is not implemented")
+        self.gradient_guided = bool(gradient_guided)
         if devices is not None and len(devices) > 1:
             raise NotImplementedError("one GPU per process: shard independent runs across processes instead")
         self.difficult_model = difficult_model
@@ -86,7 +85,7 @@ class NestedSampler:
         self.init_efficiency_threshold = init_efficiency_threshold
         self.verbose = verbose
         self.launch_kw = dict(lanes_per_chain=lanes_per_chain, waves_per_block=waves_per_block, max_steps=max_steps,
-                              lr_terms=lr_terms)
+                              lr_terms=lr_terms, gradient_guided=self.gradient_guided)
         probe = self._spec(0)
         self.num_live_points = probe.num_live_points
         self._max_samples_rounded = probe.max_samples

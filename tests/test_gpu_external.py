@@ -111,3 +111,19 @@ def test_external_two_priors_arguments():
     zb = (math.sqrt(math.pi / 2) * erf(2 / math.sqrt(2)) / 2.0) ** 2
     say("log Z", z, "analytic", math.log(za * zb))
     assert abs(z[0] - math.log(za * zb)) < 4 * z[1] + 0.1
+
+
+def test_gradient_guided_mvn_logZ():
+    """gradient_guided=True (uni_slice_sampler.py:255-271) with a fused-family likelihood (run through
+    its torch form): log Z against the analytic value, and every slice-sampled point spent at least
+    two evaluations per slice (its accepted proposal plus the gradient at it)."""
+    model, log_Z, _, _ = mvn_problem(5, seed=6)
+    ns = J.NestedSampler(model, num_live_points=150, s=3, max_samples=150 * 150, gradient_guided=True)
+    reason, state = ns(9)
+    res = ns.to_results(reason, state)
+    ev = np.asarray(res.num_likelihood_evaluations_per_sample)
+    say("log Z", res.log_Z_mean, "+-", res.log_Z_uncert, "analytic", log_Z, "min evals (slice samples)",
+        ev[ev > 1].min())
+    assert abs(res.log_Z_mean - log_Z) < 4 * res.log_Z_uncert + 0.15
+    # initial points took >= 1 evaluation; slice-sampled ones 2 per slice at least
+    assert (ev[ev > 1] >= 2 * ns.num_slices).all()
