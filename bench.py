@@ -259,31 +259,53 @@ def instrumented_slice_timing(wl, R, lpc, wpb, seed0):
     return np.array(durs), np.array(work), np.array(evals)
 
 
-def cpu_baseline(wl, spec, gpu_evals_per_run, budget_s=15.0):
-    """The oracle (a C port of the reference algorithm) on the host cores for a bounded
-    number of shells of one run of the same workload; evals/s extrapolated to runs/s with
-    the evaluations a full run needs on the device."""
+def cpu_baseline(wl, spec, gpu_evals_per_run, budget_s=12.0):
+    """The oracle (a C port of the reference algorithm) on the host cores.  A shell-limited probe
+    measures evaluations/s; when a whole run is predicted to fit twice in `budget_s`, whole runs of
+    the same workload (different seeds) are timed and runs/s is reported directly, otherwise the
+    probe is grown to about a third of the budget and extrapolated with the device's evaluations
+    per run."""
     import oracle as O
     ospec = wl.oracle_spec(spec)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    iters = 2
-    while True:
+
+    def run(seed, iters):
         t0 = time.perf_counter()
         r = O.ns_run(ospec, num_live_points=spec.num_live_points, num_slices=spec.num_slices,
-                     max_samples=spec.max_samples, seed=5, num_threads=threads, max_iterations=iters)
-        dt = time.perf_counter() - t0
-        print(f"[bench] cpu baseline: {iters} shells in {dt:.2f} s", file=sys.stderr, flush=True)
-        if dt > budget_s / 3 or iters >= 64:
+                     max_samples=spec.max_samples, seed=seed, num_threads=threads, max_iterations=iters)
+        return time.perf_counter() - t0, r["total_likelihood_evaluations"]
+
+    iters = 2
+    while True:
+        dt, ev = run(5, iters)
+        print(f"[bench] cpu baseline probe: {iters} shells in {dt:.2f} s", file=sys.stderr, flush=True)
+        if dt > 0.3 or iters >= 64:
             break
         iters *= 2
-    evals = r["total_likelihood_evaluations"]
-    eps = evals / dt
-    return dict(value=eps / gpu_evals_per_run, unit="runs/s", cores=threads, kind="port",
-                evals_per_s=eps,
+    est_run = gpu_evals_per_run / (ev / dt)
+    if 2.0 * est_run < budget_s:
+        t_all, evals, runs = 0.0, 0, 0
+        while t_all < budget_s and runs < 32:
+            dt, ev = run(11 + runs, -1)
+            t_all += dt
+            evals += ev
+            runs += 1
+            print(f"[bench] cpu baseline: run {runs} in {dt:.2f} s", file=sys.stderr, flush=True)
+        return dict(value=runs / t_all, unit="runs/s", cores=threads, kind="port", evals_per_s=evals / t_all,
+                    sample=f"{runs} whole runs of the same workload (create_init_state, shells until the "
+                           f"termination condition, to_results; D={spec.model.U_ndims}, num_live_points="
+                           f"{spec.num_live_points}, different seeds), {evals} likelihood evaluations in "
+                           f"{t_all:.2f} s")
+    while dt < budget_s / 3 and iters < 4096:
+        iters *= 2
+        dt, ev = run(5, iters)
+        print(f"[bench] cpu baseline probe: {iters} shells in {dt:.2f} s", file=sys.stderr, flush=True)
+    eps = ev / dt
+    return dict(value=eps / gpu_evals_per_run, unit="runs/s", cores=threads, kind="port", evals_per_s=eps,
                 sample=f"{iters} shells of one run of the same workload (create_init_state + {iters} x "
                        f"_collect_shell + to_results; D={spec.model.U_ndims}, num_live_points="
-                       f"{spec.num_live_points}), {evals} likelihood evals in {dt:.2f} s, "
-                       f"extrapolated with {gpu_evals_per_run:.3g} evals per full run")
+                       f"{spec.num_live_points}), {ev} likelihood evaluations in {dt:.2f} s, extrapolated "
+                       f"with {gpu_evals_per_run:.3g} evaluations per full run")
 
 
 def log(*a):
