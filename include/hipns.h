@@ -337,6 +337,11 @@ int hipns_philox4x32(const uint32_t* ctr_dev, int64_t n, uint32_t key0, uint32_t
 /* Device inverse normal CDF (the prior transform's q = ndtri), for accuracy tests. */
 int hipns_ndtri_f32(const float* u_dev, int64_t n, float* z_dev, void* stream);
 
+/* The slice bracket's quotient a / n (reciprocal + residual corrections, no operand scaling),
+ * elementwise, for tests that it equals IEEE binary32 division over the bracket's operand range
+ * (uni_slice_sampler.py:41-64 divides (0 - x) and (1 - x) by the direction). */
+int hipns_slice_quotients(const float* a_dev, const float* n_dev, int64_t count, float* q_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

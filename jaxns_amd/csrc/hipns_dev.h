@@ -40,6 +40,19 @@ __device__ __forceinline__ void swap32(float v, float& a, float& b) {
   b = __uint_as_float(r[1]);
 }
 
+// value of lane part `part` (wavefront-uniform) of this lane's chain, on every part of the chain:
+// r[0] / r[1] of a swap are the values of the lower / upper partner
+template <int LPC>
+__device__ __forceinline__ uint32_t part_broadcast(uint32_t v, int part) {
+  auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  if constexpr (LPC == 4) {
+    v = (part & 2) ? r[1] : r[0];
+    auto q = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (part & 1) ? q[1] : q[0];
+  }
+  return part ? r[1] : r[0];
+}
+
 template <int LPC>
 __device__ __forceinline__ float chain_sum(float v) {
   float a, b;
@@ -72,6 +85,25 @@ __device__ __forceinline__ float chain_max(float v) {
 }
 
 // ---------------------------------------------------------------------------
+// a / n rounded to nearest without the operand scaling of the general IEEE division: y = RN(1/n) from
+// the hardware reciprocal and one Newton step, then two residual corrections of q = a y (P. Markstein,
+// IBM J. Res. Dev. 34(1), 1990).  For |a| in [2^-60, 2] and |n| in [2^-94, 1] no scaling is needed and
+// this is the division's own sequence, so the quotient is bit-identical to a / n.  n = +-0 yields NaN
+// (where a / n is +-inf): both say "no bound from this slot".
+__device__ __forceinline__ float recip_rn(float n) {
+  const float y = __builtin_amdgcn_rcpf(n);
+  const float e = fmaf(-n, y, 1.0f);
+  return fmaf(e, y, y);
+}
+__device__ __forceinline__ float quot_rn(float a, float n, float y) {
+  float q = a * y;
+  float r = fmaf(-n, q, a);
+  q = fmaf(r, y, q);
+  r = fmaf(-n, q, a);
+  return fmaf(r, y, q);
+}
+
+// ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of the
 // multiply-xor bijection with Weyl key schedule.
 // ---------------------------------------------------------------------------
@@ -83,8 +115,11 @@ __device__ __forceinline__ u32x4 philox4x32(uint32_t c0, uint32_t c1, uint32_t c
                                             uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    // one 32x32 -> 64-bit product per multiplier gives both halves of mulhilo
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     uint32_t n0 = hi1 ^ c1 ^ k0;
     uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0;
@@ -196,36 +231,46 @@ __device__ __forceinline__ double ndtri_f64(double p) {
 
 __device__ __noinline__ float ndtri_tail_f32(float u) { return (float)ndtri_f64((double)u); }
 
-__device__ __forceinline__ float ndtri_f32(float u) {
-  // far tails (and u outside (0,1)): binary64 AS 241
-  if (!(u >= 1e-7f && u <= 0.9999999f)) return ndtri_tail_f32(u);
-  float w = -__logf(4.0f * u * (1.0f - u));
-  float x = 2.0f * u - 1.0f;
-  float p;
-  if (w < 5.0f) {
-    w = w - 2.5f;
-    p = 2.81022636e-08f;
-    p = fmaf(p, w, 3.43273939e-07f);
-    p = fmaf(p, w, -3.5233877e-06f);
-    p = fmaf(p, w, -4.39150654e-06f);
-    p = fmaf(p, w, 0.00021858087f);
-    p = fmaf(p, w, -0.00125372503f);
-    p = fmaf(p, w, -0.00417768164f);
-    p = fmaf(p, w, 0.246640727f);
-    p = fmaf(p, w, 1.50140941f);
-  } else {
-    w = __builtin_amdgcn_sqrtf(w) - 3.0f;
-    p = -0.000200214257f;
-    p = fmaf(p, w, 0.000100950558f);
-    p = fmaf(p, w, 0.00134934322f);
-    p = fmaf(p, w, -0.00367342844f);
-    p = fmaf(p, w, 0.00573950773f);
-    p = fmaf(p, w, -0.0076224613f);
-    p = fmaf(p, w, 0.00943887047f);
-    p = fmaf(p, w, 1.00167406f);
-    p = fmaf(p, w, 2.83297682f);
+// Giles' body for u in [1e-7, 1 - 1e-7].  w = -ln(4u(1-u)) comes from the hardware log2 (one
+// rounding more than a full-precision log: about 1e-7 relative in w, below the polynomials' own error).
+// The w >= 5 polynomial runs only when some lane of the wavefront needs it.
+__device__ __forceinline__ float ndtri_giles(float u) {
+  const float w = -0.693147180559945309f * __builtin_amdgcn_logf(4.0f * u * (1.0f - u));
+  const float x = 2.0f * u - 1.0f;
+  const float z = w - 2.5f;
+  float p = 2.81022636e-08f;
+  p = fmaf(p, z, 3.43273939e-07f);
+  p = fmaf(p, z, -3.5233877e-06f);
+  p = fmaf(p, z, -4.39150654e-06f);
+  p = fmaf(p, z, 0.00021858087f);
+  p = fmaf(p, z, -0.00125372503f);
+  p = fmaf(p, z, -0.00417768164f);
+  p = fmaf(p, z, 0.246640727f);
+  p = fmaf(p, z, 1.50140941f);
+  const bool tl = w >= 5.0f;
+  if (__any(tl)) {
+    const float v = __builtin_amdgcn_sqrtf(w) - 3.0f;
+    float pt = -0.000200214257f;
+    pt = fmaf(pt, v, 0.000100950558f);
+    pt = fmaf(pt, v, 0.00134934322f);
+    pt = fmaf(pt, v, -0.00367342844f);
+    pt = fmaf(pt, v, 0.00573950773f);
+    pt = fmaf(pt, v, -0.0076224613f);
+    pt = fmaf(pt, v, 0.00943887047f);
+    pt = fmaf(pt, v, 1.00167406f);
+    pt = fmaf(pt, v, 2.83297682f);
+    p = tl ? pt : p;
   }
   return 1.41421356237309505f * p * x;
+}
+
+// u far in the tails (min(u, 1 - u) < 1e-7) or outside (0, 1): binary64 AS 241
+__device__ __forceinline__ bool ndtri_far(float u) { return !(u >= 1e-7f && u <= 0.9999999f); }
+
+__device__ __forceinline__ float ndtri_f32(float u) {
+  float z = ndtri_giles(u);
+  if (ndtri_far(u)) z = ndtri_tail_f32(u);
+  return z;
 }
 
 // ---------------------------------------------------------------------------

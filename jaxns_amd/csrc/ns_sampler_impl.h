@@ -129,6 +129,36 @@ __device__ __forceinline__ const uint16_t* lr_panel_ptr(const uint16_t* x, int n
   return x + ((int64_t)(2 * gg + (h >> 1)) * npad + row) * 16 + 8 * (h & 1);
 }
 
+// q(u) of the owned slots for the prior transform: ndtri where bit j of tfm is set, the identity
+// elsewhere.  Straight-line over the slots; the binary64 far-tail path and the identity selects run
+// only when some lane of the wavefront needs them.
+template <int NJ>
+__device__ __forceinline__ void prior_q(const float (&up)[NJ], float (&q)[NJ], uint32_t tfm) {
+  if (!__any(tfm != 0u)) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) q[j] = up[j];
+    return;
+  }
+  float m = 1.0f;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const float u = up[j];
+    q[j] = ndtri_giles(u);
+    m = fminf(m, fminf(u, 1.0f - u));  // < 1e-7 exactly when ndtri_far(u), NaN aside
+  }
+  if (__any(!(m >= 1e-7f))) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (((tfm >> j) & 1u) && ndtri_far(up[j])) q[j] = ndtri_tail_f32(up[j]);
+  }
+  constexpr uint32_t all = NJ >= 32 ? 0xFFFFFFFFu : ((1u << NJ) - 1u);
+  if (__any(tfm != all)) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (!((tfm >> j) & 1u)) q[j] = up[j];
+  }
+}
+
 template <int LPC, int NJ, bool STATS = false>
 __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int D, const LinRegLds& P, uint32_t tfm,
                                              int g, int G, double log_const, double* rss = nullptr,
@@ -140,16 +170,13 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
   const int h = l / CW;
   const bool t4 = P.terms == 4;
   const bool fp32 = P.terms == 1;
-  float dwv[NJ];
+  float dwv[NJ], qv[NJ];
+  prior_q<NJ>(up, qv, tfm);
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const int d = owned_dim<LPC>(h, j);
     float dw = 0.0f;
-    if (j < nj && d < D) {
-      float u = up[j];
-      float q = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
-      dw = fmaf(P.s[d], q, P.p[d]) - P.w0[d];
-    }
+    if (j < nj && d < D) dw = fmaf(P.s[d], qv[j], P.p[d]) - P.w0[d];
     dwv[j] = dw;
   }
   // publish this wavefront's B operands, negated so that the MFMA chain started from C = r0 yields
@@ -363,17 +390,10 @@ __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, 
   const int m = l % CW;
   const int h = l / CW;
   float z[NJ];
+  prior_q<NJ>(up, z, tfm);
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) {
-    int d = owned_dim<LPC>(h, j);
-    float v = 0.0f;
-    if (j < nj) {
-      float u = up[j];
-      v = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
-      if (d >= D) v = 0.0f;
-    }
-    z[j] = v;
-  }
+  for (int j = 0; j < NJ; ++j)
+    if (!(j < nj && owned_dim<LPC>(h, j) < D)) z[j] = 0.0f;
   float ss = 0.0f;
   const int MB = (D + CW - 1) / CW;
   for (int mb = 0; mb < MB; ++mb) {
@@ -427,14 +447,14 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
   float part[HIPNS_MAX_GMM_K];
 #pragma unroll
   for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) part[k] = 0.0f;
+  float qv[NJ];
+  prior_q<NJ>(up, qv, tfm);
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     if (j < nj) {
       int d = owned_dim<LPC>(h, j);
       if (d < D) {
-        float u = up[j];
-        float qv = ((tfm >> j) & 1u) ? ndtri_f32(u) : u;
-        float x = fmaf(P.q[d], qv, P.p[d]);
+        float x = fmaf(P.q[d], qv[j], P.p[d]);
 #pragma unroll
         for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
           if (k < P.K) {
@@ -561,9 +581,9 @@ __device__ __forceinline__ void slice_direction(float (&nv)[NJ], int chain, uint
       for (int q = 0; q < 4; ++q) {
         float u1 = (float)((ws[q] & 0xFFFFu) + 1u) * 1.52587890625e-5f;  // (0, 1]
         float u2 = (float)(ws[q] >> 16) * 1.52587890625e-5f;             // [0, 1)
-        float rr = __builtin_amdgcn_sqrtf(-2.0f * __logf(u1));
-        float sn, cs;
-        __sincosf(6.28318530717958648f * u2, &sn, &cs);
+        // sqrt(-2 ln u1) with the hardware log2; sin / cos of 2 pi u2 take the angle in revolutions
+        float rr = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u1));
+        float sn = __builtin_amdgcn_sinf(u2), cs = __builtin_amdgcn_cosf(u2);
         float z0 = d0 + 2 * q < D ? rr * cs : 0.0f;
         float z1 = d0 + 2 * q + 1 < D ? rr * sn : 0.0f;
         nv[j + 2 * q] = z0;
@@ -584,11 +604,35 @@ __device__ __forceinline__ void slice_direction(float (&nv)[NJ], int chain, uint
 
 // _slice_bounds (uni_slice_sampler.py:41-64).  Padding slots (beyond D or nj) hold U0 = 0.5 and n = +0,
 // whose quotients are +-inf and never the bound, so every slot is taken unconditionally.
+// With every coordinate inside (2^-60, 1 - 2^-60) the numerators 1 - u0 > 0 > -u0 are nonzero, so the
+// sign of n alone says which quotient bounds which side, and both quotients share one reciprocal.  A
+// unit direction has a component of size >= 1/sqrt(D), which bounds the bracket by sqrt(D); slots with
+// |n| < 2^-94 (quotients beyond 2^34) can therefore never be the bound.  A wavefront holding a
+// coordinate on or outside that range takes the general candidate rule with full divisions.
 template <int LPC, int NJ>
 __device__ __forceinline__ void slice_bracket(const float (&U0)[NJ], const float (&nv)[NJ], float& left, float& right) {
+#pragma clang fp contract(off)
   float rmin = __builtin_inff(), lmax = -__builtin_inff();
+  // smallest numerator, compared as signed bit patterns (the order of non-negative floats; a negative
+  // numerator is below every positive one)
+  int amin = 0x3F800000;
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) bound_candidates(U0[j], nv[j], rmin, lmax);
+  for (int j = 0; j < NJ; ++j) {
+    const float u0 = U0[j], n = nv[j];
+    const float a1 = 1.0f - u0, a0 = -u0;
+    amin = min(amin, min(__float_as_int(u0), __float_as_int(a1)));
+    const float y = recip_rn(n);
+    const float t1 = quot_rn(a1, n, y), t0 = quot_rn(a0, n, y);
+    const bool pos = n > 0.0f;
+    rmin = fminf(rmin, pos ? t1 : t0);
+    lmax = fmaxf(lmax, pos ? t0 : t1);
+  }
+  if (__any(amin < 0x21800000)) {  // 0x21800000 = 2^-60
+    rmin = __builtin_inff();
+    lmax = -__builtin_inff();
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) bound_candidates(U0[j], nv[j], rmin, lmax);
+  }
   right = chain_min<LPC>(rmin);
   left = chain_max<LPC>(lmax);
 }
@@ -727,11 +771,19 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
 
   // Every live chain evaluates exactly once per step, so the step index w equals each live
   // chain's evaluation count: the pick uniform of a chain's w-th evaluation is word (w mod 4)
-  // of its Philox block w/4, drawn in the same step by the whole wavefront.
+  // of its Philox block w/4.  The LPC lanes of a chain draw LPC consecutive blocks at once (part h
+  // block w/4 + h, every 4 LPC steps), and each step the lane part holding block w/4 passes its
+  // word to the others.
   for (int w = 0; COOP ? __syncthreads_or(alive) : __any(alive); ++w) {
+    uint32_t pickw = 0u;
+    if (MODE == MODE_SLICE) {
+      if ((w & (4 * LPC - 1)) == 0)
+        pickblk = philox4x32((uint32_t)chain, it32, (uint32_t)((w >> 2) + h), RNG_PICK, k0, k1);
+      const int kw = w & 3;
+      const uint32_t own = kw == 0 ? pickblk.x : (kw == 1 ? pickblk.y : (kw == 2 ? pickblk.z : pickblk.w));
+      pickw = part_broadcast<LPC>(own, (w >> 2) & (LPC - 1));
+    }
     if (MODE == MODE_SLICE || MODE == MODE_INJ) {
-      if (MODE == MODE_SLICE && (w & 3) == 0)
-        pickblk = philox4x32((uint32_t)chain, it32, (uint32_t)(w >> 2), RNG_PICK, k0, k1);
       if (alive) {
         if (need_begin) {
           begin_slice(s);
@@ -746,8 +798,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
         if (MODE == MODE_INJ) {
           u = A.inj_tu[((int64_t)chain * num_slices + s) * A.max_steps + k];
         } else {
-          const int kw = w & 3;
-          u = u01f(kw == 0 ? pickblk.x : (kw == 1 ? pickblk.y : (kw == 2 ? pickblk.z : pickblk.w)));
+          u = u01f(pickw);
         }
         // _pick_point_in_interval (uni_slice_sampler.py:67-89)
         t = pick_t(left, right, u);
