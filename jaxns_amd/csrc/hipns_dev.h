@@ -236,7 +236,7 @@ __device__ __noinline__ float ndtri_tail_f32(float u) { return (float)ndtri_f64(
 // The w >= 5 polynomial runs only when some lane of the wavefront needs it.
 __device__ __forceinline__ float ndtri_giles(float u) {
   const float w = -0.693147180559945309f * __builtin_amdgcn_logf(4.0f * u * (1.0f - u));
-  const float x = 2.0f * u - 1.0f;
+  const float x = fmaf(u, 2.82842712474619010f, -1.41421356237309505f);  // sqrt(2) (2u - 1), one rounding
   const float z = w - 2.5f;
   float p = 2.81022636e-08f;
   p = fmaf(p, z, 3.43273939e-07f);
@@ -261,7 +261,7 @@ __device__ __forceinline__ float ndtri_giles(float u) {
     pt = fmaf(pt, v, 2.83297682f);
     p = tl ? pt : p;
   }
-  return 1.41421356237309505f * p * x;
+  return p * x;
 }
 
 // u far in the tails (min(u, 1 - u) < 1e-7) or outside (0, 1): binary64 AS 241

@@ -72,6 +72,13 @@ __host__ __device__ inline int sampler_nj(int dmax, int lpc) {
   return dmax <= 32 ? 8 : (dmax <= 64 ? 16 : 32);
 }
 
+// MVN: row stride of the transposed whitening matrix in LDS for the instance (NJ, LPC): the instance's
+// full width NJ * LPC in blocks of CW = 64 / LPC output rows, + 1 against bank conflicts.  A
+// compile-time stride lets the kernel address every A operand with an immediate offset.
+__host__ __device__ constexpr int mvn_row_stride(int nj, int lpc) {
+  return ((nj * lpc + 64 / lpc - 1) / (64 / lpc)) * (64 / lpc) + 1;
+}
+
 // floats of dynamic LDS one sampler block needs (layout of load_params in ns_sampler_impl.h)
 inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc, int G = 1) {
   if (fam == HIPNS_FAM_LINREG) {
@@ -80,10 +87,10 @@ inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc, int G 
     return (size_t)4 * dps + 4 + (size_t)G * NG * 3 * 64 * 4 + (size_t)2 * G * G * CW * 2;
   }
   if (fam == HIPNS_FAM_MVN) {
-    int CW = 64 / lpc;
-    int MB = (D + CW - 1) / CW;
-    int RS = MB * CW + 1;
-    return (size_t)2 * dps + (size_t)dps * RS;
+    // W^T holds a (zero) row for every dimension slot of the instance, so the kernel reads all NJ slots
+    const int nj = sampler_nj(D, lpc);
+    const int rows = dps > nj * lpc ? dps : nj * lpc;
+    return (size_t)2 * dps + (size_t)rows * mvn_row_stride(nj, lpc);
   }
   return (size_t)(3 + K) * dps + 2 * K;
 }

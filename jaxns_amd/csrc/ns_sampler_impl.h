@@ -139,14 +139,16 @@ __device__ __forceinline__ void prior_q(const float (&up)[NJ], float (&q)[NJ], u
     for (int j = 0; j < NJ; ++j) q[j] = up[j];
     return;
   }
-  float m = 1.0f;
+  // smallest of u and 1 - u as signed bit patterns (the order of non-negative floats, negatives below):
+  // below the pattern of 1e-7f exactly when ndtri_far(u), NaN aside
+  int m = 0x3F800000;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const float u = up[j];
     q[j] = ndtri_giles(u);
-    m = fminf(m, fminf(u, 1.0f - u));  // < 1e-7 exactly when ndtri_far(u), NaN aside
+    m = min(m, min(__float_as_int(u), __float_as_int(1.0f - u)));
   }
-  if (__any(!(m >= 1e-7f))) {
+  if (__any(m < 0x33D6BF95)) {  // 0x33D6BF95 = 1e-7f
 #pragma unroll
     for (int j = 0; j < NJ; ++j)
       if (((tfm >> j) & 1u) && ndtri_far(up[j])) q[j] = ndtri_tail_f32(up[j]);
@@ -394,42 +396,50 @@ __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, 
 #pragma unroll
   for (int j = 0; j < NJ; ++j)
     if (!(j < nj && owned_dim<LPC>(h, j) < D)) z[j] = 0.0f;
+  // A operands W[row][owned dim] at compile-time offsets from the lane's base (row m of block mb, owned
+  // dim 8h); the accumulator of each block starts from this lane's output rows of b.  W^T has a row for
+  // every slot of the instance (zero beyond the model), and z is zero on slots beyond D, so all NJ
+  // slots are multiplied unconditionally.
+  constexpr int RS = mvn_row_stride(NJ, LPC);
+  constexpr int MBI = (NJ * LPC + CW - 1) / CW;
+  const float* wt = P.WT + 8 * h * RS + m;
   float ss = 0.0f;
   const int MB = (D + CW - 1) / CW;
-  for (int mb = 0; mb < MB; ++mb) {
-    const int row = mb * CW + m;
+#pragma unroll
+  for (int mb = 0; mb < MBI; ++mb) {
+    if (mb >= MB) break;
     if constexpr (LPC == 2) {
-      f32x16 acc0 = {0}, acc1 = {0};
+      f32x16 acc0, acc1 = {0};
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const f32x4 bb = *(const f32x4*)(P.b + mb * 32 + 8 * g + 4 * h);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc0[4 * g + i] = bb[i];
+      }
 #pragma unroll
       for (int j = 0; j < NJ; j += 2) {
-        if (j < nj) {
-          float a0 = P.WT[owned_dim<LPC>(h, j) * P.RS + row];
-          acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, z[j], acc0, 0, 0, 0);
-          float a1 = P.WT[owned_dim<LPC>(h, j + 1) * P.RS + row];
-          acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, z[j + 1], acc1, 0, 0, 0);
-        }
+        const float a0 = wt[(8 * LPC * (j >> 3) + (j & 7)) * RS + mb * CW];
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, z[j], acc0, 0, 0, 0);
+        const float a1 = wt[(8 * LPC * ((j + 1) >> 3) + ((j + 1) & 7)) * RS + mb * CW];
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, z[j + 1], acc1, 0, 0, 0);
       }
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        int r = mb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-        float v = acc0[i] + acc1[i] + P.b[r];
+        const float v = acc0[i] + acc1[i];
         ss = fmaf(v, v, ss);
       }
     } else {
-      f32x4 acc0 = {0}, acc1 = {0};
+      f32x4 acc0 = *(const f32x4*)(P.b + mb * 16 + 4 * h), acc1 = {0};
 #pragma unroll
       for (int j = 0; j < NJ; j += 2) {
-        if (j < nj) {
-          float a0 = P.WT[owned_dim<LPC>(h, j) * P.RS + row];
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, z[j], acc0, 0, 0, 0);
-          float a1 = P.WT[owned_dim<LPC>(h, j + 1) * P.RS + row];
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, z[j + 1], acc1, 0, 0, 0);
-        }
+        const float a0 = wt[(8 * LPC * (j >> 3) + (j & 7)) * RS + mb * CW];
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, z[j], acc0, 0, 0, 0);
+        const float a1 = wt[(8 * LPC * ((j + 1) >> 3) + ((j + 1) & 7)) * RS + mb * CW];
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, z[j + 1], acc1, 0, 0, 0);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        int r = mb * 16 + 4 * h + i;
-        float v = acc0[i] + acc1[i] + P.b[r];
+        const float v = acc0[i] + acc1[i];
         ss = fmaf(v, v, ss);
       }
     }
@@ -513,9 +523,7 @@ __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, 
     return;
   }
   if (fam == HIPNS_FAM_MVN) {
-    const int CW = HIPNS_WAVE / LPCv;
-    const int MB = (D + CW - 1) / CW;
-    const int RS = MB * CW + 1;
+    const int RS = mvn_row_stride(8 * NG, LPCv);
     mv.tf = lds;
     mv.b = lds + dps;
     mv.WT = lds + 2 * dps;
@@ -525,9 +533,10 @@ __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, 
     }
     for (int i = tid; i < dps; i += nt) mv.b[i] = prm[dps + i];
     const float* W = prm + 2 * dps;
-    for (int i = tid; i < dps * RS; i += nt) {
+    const int rows = dps > 8 * NG * LPCv ? dps : 8 * NG * LPCv;
+    for (int i = tid; i < rows * RS; i += nt) {
       int d = i / RS, m = i % RS;
-      mv.WT[i] = (m < dps && m < MB * CW) ? W[m * dps + d] : 0.0f;
+      mv.WT[i] = (d < dps && m < dps && m < RS - 1) ? W[m * dps + d] : 0.0f;
     }
   } else {
     gm.tf = lds;
