@@ -160,13 +160,40 @@ size_t replace_lds_bytes(int N) {
   return (size_t)P * 8 + (size_t)P * 4 + (size_t)N * 4;
 }
 
-// ---------------------------------------------------------------------------
-// ordered block reduction of evidence maps ("first then second")
-// ---------------------------------------------------------------------------
-__device__ EvMap block_reduce_maps(EvMap m, EvMap* lds_maps) {
+// Aggregate maps of two element sequences at once: waves [0, w1) of the block cover the first
+// (n_i = N, the discarded shell), waves [w1, nw) the second (n_i = N - i, the remaining live points).
+// Each thread composes a contiguous run of its sequence, each wavefront reduces its lanes in order,
+// and a pairwise tree over the wavefront maps of each sequence (both trees in the same passes)
+// finishes.  The serial depth is a few compositions per thread plus log2 of the lane and wave counts.
+// Returns through lds_maps[0] (first sequence) and lds_maps[w1] (second).
+struct SeqSpec {
+  const double* L;
+  int n;
+  double nconst;  // > 0: every n_i = nconst; else n_i = ndesc - i
+  int ndesc;
+  double L0;      // Lprev of element 0
+};
+
+__device__ EvMap thread_map(const SeqSpec& q, int tt, int nthreads) {
+  const int per = (q.n + nthreads - 1) / nthreads;
+  const int b = tt * per;
+  const int e = min(q.n, b + per);
+  EvMap m = evmap_identity();
+  for (int i = b; i < e; ++i) {
+    double Lp = i == 0 ? q.L0 : q.L[i - 1];
+    double ni = q.nconst > 0.0 ? q.nconst : (double)(q.ndesc - i);
+    double lt;
+    m = evmap_then(m, evmap_step(ni, Lp, q.L[i], &lt));
+  }
+  return m;
+}
+
+__device__ void two_sequence_maps(const SeqSpec& q1, const SeqSpec& q2, int w1, EvMap* lds_maps) {
   const int l = lane_id();
   const int w = threadIdx.x / HIPNS_WAVE;
   const int nw = blockDim.x / HIPNS_WAVE;
+  EvMap m = (w < w1) ? thread_map(q1, threadIdx.x, w1 * HIPNS_WAVE)
+                     : thread_map(q2, threadIdx.x - w1 * HIPNS_WAVE, (nw - w1) * HIPNS_WAVE);
 #pragma unroll
   for (int o = 1; o < HIPNS_WAVE; o <<= 1) {
     EvMap other = {__shfl_down(m.a, o, 64), __shfl_down(m.b, o, 64), __shfl_down(m.g, o, 64),
@@ -176,27 +203,17 @@ __device__ EvMap block_reduce_maps(EvMap m, EvMap* lds_maps) {
   }
   if (l == 0) lds_maps[w] = m;
   __syncthreads();
-  EvMap acc = evmap_identity();
-  if (threadIdx.x == 0)
-    for (int i = 0; i < nw; ++i) acc = evmap_then(acc, lds_maps[i]);
-  __syncthreads();
-  return acc;  // valid in thread 0
-}
-
-// aggregate map of elements [0, n): n_i = nconst (if > 0) else (ndesc - i); Lprev of element 0 is L0
-__device__ EvMap chunk_map(const double* L, int n, double nconst, int ndesc, double L0, EvMap* lds_maps) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int per = (n + nt - 1) / nt;
-  const int b = tid * per;
-  const int e = min(n, b + per);
-  EvMap m = evmap_identity();
-  for (int i = b; i < e; ++i) {
-    double Lp = i == 0 ? L0 : L[i - 1];
-    double ni = nconst > 0.0 ? nconst : (double)(ndesc - i);
-    double lt;
-    m = evmap_then(m, evmap_step(ni, Lp, L[i], &lt));
+  for (int span = 1; span < nw; span <<= 1) {
+    const int t = threadIdx.x;
+    if (t < nw) {
+      const int base = t < w1 ? 0 : w1, cnt = t < w1 ? w1 : nw - w1, i = t - base;
+      if ((i % (2 * span)) == 0 && i + span < cnt) {
+        const EvMap a = lds_maps[t], b = lds_maps[t + span];
+        lds_maps[t] = evmap_then(a, b);
+      }
+    }
+    __syncthreads();
   }
-  return block_reduce_maps(m, lds_maps);
 }
 
 // ---------------------------------------------------------------------------
@@ -216,10 +233,12 @@ __global__ void __launch_bounds__(1024) k_register(const hipns_run_desc_t* runs,
   const double* disc = B.live_logL[cur] + run.live_off;
   const double* live = B.live_logL[nxt] + run.live_off;
   const EvState s0 = evstate_from(st->evidence_calc);
-  // discarded shell with n = N (sharded_static.py:284-291)
-  EvMap A1 = chunk_map(disc, S, (double)N, 0, s0.L, lds_maps);
-  // remaining live points with n = N, N-1, ..., 1 (sharded_static.py:292-299)
-  EvMap A2 = chunk_map(live, N, 0.0, N, disc[S - 1], lds_maps);
+  // discarded shell with n = N (sharded_static.py:284-291) and the remaining live points with
+  // n = N, N-1, ..., 1 (sharded_static.py:292-299), wavefronts split in proportion to their lengths
+  const int nw = blockDim.x / HIPNS_WAVE;
+  const int w1 = max(1, min(nw - 1, (int)(((int64_t)nw * S + (S + N) / 2) / (S + N))));
+  two_sequence_maps(SeqSpec{disc, S, (double)N, 0, s0.L}, SeqSpec{live, N, 0.0, N, disc[S - 1]}, w1, lds_maps);
+  const EvMap A1 = lds_maps[0], A2 = lds_maps[w1];
   // live evaluation counts, new evaluation counts, plateau
   long long le = 0, ne = 0;
   int plat = 1;
@@ -234,7 +253,7 @@ __global__ void __launch_bounds__(1024) k_register(const hipns_run_desc_t* runs,
     ne += __shfl_down(ne, o, 64);
     plat &= __shfl_down(plat, o, 64);
   }
-  const int w = threadIdx.x / 64, nw = blockDim.x / 64;
+  const int w = threadIdx.x / 64;
   if (lane_id() == 0) {
     red_i[w] = le;
     red_i[16 + w] = ne;
