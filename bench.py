@@ -14,7 +14,7 @@ Other workloads:
   c3  256 runs of uneven D in {4..64} and num_live_points in {128..1024},
       4-component isotropic Gaussian mixture, prior U[-10, 10]^D
   c4  Bayesian linear regression, 4096 rows x 16 features, sigma=0.3,
-      w ~ N(0, I), num_live_points=1024, s=4; residual GEMM in BF16x3 MFMA
+      w ~ N(0, I), num_live_points=1024, s=5; residual GEMM in BF16x3 MFMA
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects for the
 kernel-level measurement and the host-CPU reference).
@@ -103,7 +103,7 @@ class GmmBatchWorkload(Workload):
 
     def __init__(self):
         self.name = ("256 runs, D~U{4..64}, num_live_points~U{128..1024}, 4-component isotropic Gaussian mixture, "
-                     "prior U[-10,10]^D, s=4 (FP32 likelihood, FP64 evidence)")
+                     "prior U[-10,10]^D, s=5 (FP32 likelihood, FP64 evidence)")
         self._cache = {}
 
     def _run(self, seed):
@@ -121,7 +121,7 @@ class GmmBatchWorkload(Workload):
         sp, tr = [], []
         for sd in seeds:
             D, N, model, log_Z, _, _ = self._run(sd)
-            sp.append(RunSpec(model=model, num_live_points=N, num_slices=4 * D, k_phantom=0, max_samples=N * 300,
+            sp.append(RunSpec(model=model, num_live_points=N, num_slices=5 * D, k_phantom=0, max_samples=N * 300,
                               seed=sd))
             tr.append(log_Z)
         return sp, np.array(tr)
@@ -142,7 +142,7 @@ class LinRegWorkload(Workload):
     dtype = "bf16x3 likelihood GEMM (MFMA, f32 accumulate) + f64 evidence"
     peak = PEAK_BF16_MFMA
 
-    def __init__(self, Ndata=4096, D=16, N=1024, s=4, sigma=0.3, tau=1.0, lr_terms=3):
+    def __init__(self, Ndata=4096, D=16, N=1024, s=5, sigma=0.3, tau=1.0, lr_terms=3):
         from problems import linreg_problem
         self.Ndata, self.D, self.N, self.s, self.tau = Ndata, D, N, s, tau
         self.lr_terms = lr_terms
