@@ -236,15 +236,19 @@ def test_batched_loglik_vs_oracle(fam, D, lpc, wpb, nd):
             assert post.max() < 1e-6 * np.abs(ref[B // 2:]).max() + 2e-3
 
 
-@pytest.mark.parametrize("fam,D,lpc,k,wpb", [("mvn", 10, 2, 0, 2), ("mvn", 32, 2, 2, 2), ("mvn", 32, 4, 0, 2),
-                                             ("mvn", 5, 4, 1, 2), ("linreg", 16, 2, 1, 4), ("linreg", 40, 4, 0, 3)])
-def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb):
+@pytest.mark.parametrize("fam,D,lpc,k,wpb,faces", [("mvn", 10, 2, 0, 2, False), ("mvn", 32, 2, 2, 2, False),
+                                                   ("mvn", 32, 4, 0, 2, False), ("mvn", 5, 4, 1, 2, False),
+                                                   ("linreg", 16, 2, 1, 4, False), ("linreg", 40, 4, 0, 3, False),
+                                                   ("mvn", 10, 2, 0, 2, True), ("mvn", 5, 4, 0, 1, True)])
+def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb, faces):
     """UniDimSliceSampler chains with injected draws: the device trace (t of every pick),
     the accept/reject sequence, eval counts and the accepted points are bit-exact with
     the binary32 oracle replaying the device's likelihood values; the device likelihood
-    itself agrees with the FP64 oracle to 1e-5."""
+    itself agrees with the FP64 oracle to 1e-5.  faces=True seeds chains at points with a
+    coordinate within 2^-60 of a face of the unit cube (down to the smallest subnormal), where the
+    bracket takes the general candidate rule, and just inside that range."""
     torch = _torch()
-    say(f"slice injected {fam} D={D} lpc={lpc} k={k} wpb={wpb}")
+    say(f"slice injected {fam} D={D} lpc={lpc} k={k} wpb={wpb} faces={faces}")
     if fam == "mvn":
         model, _, mu, Sig = mvn_problem(D, seed=100 + D)
         spec = mvn_oracle_spec(D, mu, Sig)
@@ -256,12 +260,23 @@ def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb):
     b = _batch(model, N=N, num_slices=ns, k=k, seed=7, lpc=lpc, wpb=wpb)
     b.init_state()
     torch.cuda.synchronize()
+    # (a coordinate exactly on a face collapses the bracket to t = 0, which the reference's loop never
+    # leaves unless the seed itself is accepted, so the cases stop just inside the faces)
+    face_vals = [1e-30, 2.0 ** -70, 2.0 ** -59, 2.0 ** -149, 1.0 - 2.0 ** -24, 2.0 ** -61]
+    if faces:
+        for i, v in enumerate(face_vals):
+            b.live_U[0][N - 1 - i, i % D] = v
+        torch.cuda.synchronize()
     live_U = b.live_U[0][:N, :D].cpu().numpy()
     live_L = b.live_logL[0][:N].cpu().numpy()
-    contour = float(live_L[N // 2 - 1])
+    # near-face seeds no longer satisfy the contour they were drawn under; with contour = -inf every
+    # slice accepts its first pick, so each slice tests one bracket (the first from the face point)
+    contour = -np.inf if faces else float(live_L[N // 2 - 1])
     rng = np.random.default_rng(D + lpc)
     Cn = N // 2
     seeds = np.array([O.seed_index(live_L, contour, u) for u in rng.uniform(size=Cn)], np.int32)
+    if faces:
+        seeds[:len(face_vals)] = N - 1 - np.arange(len(face_vals))
     max_steps = 64
     z = rng.normal(size=(Cn, ns, D))
     dirs = (z / np.linalg.norm(z, axis=-1, keepdims=True)).astype(np.float32)
@@ -304,10 +319,15 @@ def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb):
                             dtype=np.float32, logL_replay=None)
         if np.array_equal(o64["steps"], o["steps"]):
             mm = ~np.isnan(o64["trace_logL"])
-            rel = np.abs(o64["trace_logL"][mm] - tr_L[c][mm]) / np.maximum(np.abs(o64["trace_logL"][mm]), 1.0)
-            assert rel.max() < 1e-5
+            ref, got = o64["trace_logL"][mm], tr_L[c][mm]
+            fin = np.isfinite(ref)
+            assert np.array_equal(ref[~fin], got[~fin])  # points on a face: logL = -inf on both sides
+            rel = np.abs(ref[fin] - got[fin]) / np.maximum(np.abs(ref[fin]), 1.0)
+            assert rel.size == 0 or rel.max() < 1e-5
     # accepted points satisfy the contour (uni_slice_sampler.py:160-166)
     assert np.all(g_L > contour)
+    if faces:
+        assert np.all(steps == 1)
 
 
 def test_full_run_config1_logZ():
