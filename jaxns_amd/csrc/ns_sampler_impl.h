@@ -161,6 +161,130 @@ __device__ __forceinline__ void prior_q(const float (&up)[NJ], float (&q)[NJ], u
   }
 }
 
+template <int LPC>
+struct LrAcc;
+template <>
+struct LrAcc<2> {
+  typedef f32x16 type;  // v_mfma_f32_32x32x16_bf16: 32 rows x 32 chains, 16 rows per lane
+};
+template <>
+struct LrAcc<4> {
+  typedef f32x4 type;  // v_mfma_f32_16x16x32_bf16: 16 rows x 16 chains, 4 rows per lane
+};
+
+template <int LPC>
+__device__ __forceinline__ typename LrAcc<LPC>::type lr_mfma(const bf16x8_t& a, const bf16x8_t& b,
+                                                              const typename LrAcc<LPC>::type& c) {
+  if constexpr (LPC == 2)
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// The BF16 residual tiles [t0, t1) of one wavefront against the B operands of GG chain groups (GG
+// known at compile time, so the groups' MFMA chains are interleaved and nothing branches per group).
+// Tiles are double-buffered with wavefront-uniform bounds: the next tile's X panels and r0 load while
+// this one multiplies.  With few (group, k-group) pairs the step's B operands are held in registers
+// for all tiles; otherwise they are read from LDS per tile.
+template <int LPC, int NJ, int GG, bool STATS>
+__device__ __forceinline__ void lr_bf16_tiles(const LinRegLds& P, const bf16x8_t* Bs, int t0, int t1, bool t4,
+                                              double (&ss)[LR_MAX_G], double (&sy)[LR_MAX_G]) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  constexpr int NG = NJ / 8;
+  constexpr int NR = (LPC == 2) ? 16 : 4;
+  constexpr bool BREG = LPC == 4 && GG * NG <= 4;
+  typedef typename LrAcc<LPC>::type Acc;
+  const int l = lane_id();
+  const int m = l % CW;
+  const int h = l / CW;
+  auto bop = [&](int q, int gg, int part) -> bf16x8_t { return Bs[((q * NG + gg) * 3 + part) * HIPNS_WAVE + l]; };
+  bf16x8_t bh[BREG ? GG : 1][NG], bl[BREG ? GG : 1][NG];
+  if constexpr (BREG) {
+#pragma unroll
+    for (int q = 0; q < GG; ++q)
+#pragma unroll
+      for (int gg = 0; gg < NG; ++gg) {
+        bh[q][gg] = bop(q, gg, 0);
+        bl[q][gg] = bop(q, gg, 1);
+      }
+  }
+  auto tile_row = [&](int tl, int i) { return (LPC == 2) ? tl * 32 + 8 * (i >> 2) + 4 * h + (i & 3) : tl * 16 + 4 * h + i; };
+  auto load = [&](int tl, bf16x8_t (&xh)[NG], bf16x8_t (&xl)[NG], float (&rv)[NR]) {
+    const int row = tl * CW + m;
+#pragma unroll
+    for (int gg = 0; gg < NG; ++gg) {
+      xh[gg] = __builtin_bit_cast(bf16x8_t, *(const uint4*)lr_panel_ptr<LPC>(P.xh, P.npad, gg, row, h));
+      xl[gg] = __builtin_bit_cast(bf16x8_t, *(const uint4*)lr_panel_ptr<LPC>(P.xl, P.npad, gg, row, h));
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) rv[i] = P.r0[tile_row(tl, i)];
+  };
+  // groups per pass over a tile: LPC=2 accumulators are 16 VGPRs each, so two at a time
+  constexpr int QB = (LPC == 2 && GG > 2) ? 2 : GG;
+  auto compute = [&](int tl, const bf16x8_t (&xh)[NG], const bf16x8_t (&xl)[NG], const float (&rv)[NR]) {
+    float yv[NR];
+    if (STATS) {
+#pragma unroll
+      for (int i = 0; i < NR; ++i) yv[i] = P.y[tile_row(tl, i)];
+    }
+#pragma unroll
+    for (int q0 = 0; q0 < GG; q0 += QB) {
+      constexpr int QN = QB;
+      Acc acc[QN];
+#pragma unroll
+      for (int qq = 0; qq < QN; ++qq)
+#pragma unroll
+        for (int i = 0; i < NR; ++i) acc[qq][i] = rv[i];
+#pragma unroll
+      for (int gg = 0; gg < NG; ++gg) {
+        // hi.hi, hi.lo, lo.hi of every group of the pass in turn: consecutive MFMAs are independent
+#pragma unroll
+        for (int qq = 0; qq < QN; ++qq)
+          if (q0 + qq < GG)
+            acc[qq] = lr_mfma<LPC>(xh[gg], BREG ? bh[q0 + qq][gg] : bop(q0 + qq, gg, 0), acc[qq]);
+#pragma unroll
+        for (int qq = 0; qq < QN; ++qq)
+          if (q0 + qq < GG)
+            acc[qq] = lr_mfma<LPC>(xh[gg], BREG ? bl[q0 + qq][gg] : bop(q0 + qq, gg, 1), acc[qq]);
+#pragma unroll
+        for (int qq = 0; qq < QN; ++qq)
+          if (q0 + qq < GG)
+            acc[qq] = lr_mfma<LPC>(xl[gg], BREG ? bh[q0 + qq][gg] : bop(q0 + qq, gg, 0), acc[qq]);
+        if (t4) {
+#pragma unroll
+          for (int qq = 0; qq < QN; ++qq)
+            if (q0 + qq < GG) acc[qq] = lr_mfma<LPC>(xh[gg], bop(q0 + qq, gg, 2), acc[qq]);
+        }
+      }
+#pragma unroll
+      for (int qq = 0; qq < QN; ++qq) {
+        if (q0 + qq < GG) {
+          // four independent partial sums of the squared residuals (no serial FMA chain)
+          float pp[4] = {0.0f, 0.0f, 0.0f, 0.0f}, py = 0.0f;
+#pragma unroll
+          for (int i = 0; i < NR; ++i) {
+            const float r = acc[qq][i];
+            pp[i & 3] = fmaf(r, r, pp[i & 3]);
+            if (STATS) py = fmaf(yv[i], r, py);
+          }
+          ss[q0 + qq] += (double)((pp[0] + pp[1]) + (pp[2] + pp[3]));
+          if (STATS) sy[q0 + qq] += (double)py;
+        }
+      }
+    }
+  };
+  bf16x8_t xh0[NG], xl0[NG], xh1[NG], xl1[NG];
+  float r00[NR], r01[NR];
+  if (t0 < t1) load(t0, xh0, xl0, r00);
+  for (int tl = t0; tl < t1; tl += 2) {
+    if (tl + 1 < t1) load(tl + 1, xh1, xl1, r01);
+    compute(tl, xh0, xl0, r00);
+    if (tl + 1 >= t1) break;
+    if (tl + 2 < t1) load(tl + 2, xh0, xl0, r00);
+    compute(tl + 1, xh1, xl1, r01);
+  }
+}
+
 template <int LPC, int NJ, bool STATS = false>
 __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int D, const LinRegLds& P, uint32_t tfm,
                                              int g, int G, double log_const, double* rss = nullptr,
@@ -281,81 +405,16 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
       }
     }
   } else {
-  // software pipeline: the X panels and r0 of the next tile are in flight while this one multiplies.
-  // All NG k-groups are read: the panels cover NJ * LPC columns, the unused ones are zero.
-  bf16x8_t ah[NG], al[NG], nh[NG], nl[NG];
-  float r0v[NR], nr0[NR];
-  auto load_tile = [&](int tl, bf16x8_t (&xh)[NG], bf16x8_t (&xl)[NG], float (&rv)[NR]) {
-    const int row = tl * CW + m;
-#pragma unroll
-    for (int gg = 0; gg < NG; ++gg) {
-      xh[gg] = __builtin_bit_cast(bf16x8_t, *(const uint4*)lr_panel_ptr<LPC>(P.xh, P.npad, gg, row, h));
-      xl[gg] = __builtin_bit_cast(bf16x8_t, *(const uint4*)lr_panel_ptr<LPC>(P.xl, P.npad, gg, row, h));
-    }
-#pragma unroll
-    for (int i = 0; i < NR; ++i) rv[i] = P.r0[tile_row(tl, i)];
-  };
-  if (t0 < t1) load_tile(t0, ah, al, r0v);
-  for (int tl = t0; tl < t1; ++tl) {
-    if (tl + 1 < t1) load_tile(tl + 1, nh, nl, nr0);
-    float yv[NR];
-    if (STATS) {
-#pragma unroll
-      for (int i = 0; i < NR; ++i) yv[i] = P.y[tile_row(tl, i)];
-    }
-#pragma unroll
-    for (int q = 0; q < LR_MAX_G; ++q) {
-      if (q < G) {
-        float part = 0.0f, ypart = 0.0f;
-        if constexpr (LPC == 2) {
-          f32x16 acc;
-#pragma unroll
-          for (int i = 0; i < 16; ++i) acc[i] = r0v[i];
-#pragma unroll
-          for (int gg = 0; gg < NG; ++gg) {
-            const bf16x8_t* B = Bs + ((q * NG + gg) * 3) * HIPNS_WAVE + l;
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[gg], B[0], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[gg], B[HIPNS_WAVE], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[gg], B[0], acc, 0, 0, 0);
-            if (t4) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[gg], B[2 * HIPNS_WAVE], acc, 0, 0, 0);
-          }
-#pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const float r = acc[i];
-            part = fmaf(r, r, part);
-            if (STATS) ypart = fmaf(yv[i], r, ypart);
-          }
-        } else {
-          f32x4 acc;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) acc[i] = r0v[i];
-#pragma unroll
-          for (int gg = 0; gg < NG; ++gg) {
-            const bf16x8_t* B = Bs + ((q * NG + gg) * 3) * HIPNS_WAVE + l;
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[gg], B[0], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[gg], B[HIPNS_WAVE], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[gg], B[0], acc, 0, 0, 0);
-            if (t4) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[gg], B[2 * HIPNS_WAVE], acc, 0, 0, 0);
-          }
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const float r = acc[i];
-            part = fmaf(r, r, part);
-            if (STATS) ypart = fmaf(yv[i], r, ypart);
-          }
-        }
-        ss[q] += (double)part;
-        if (STATS) sy[q] += (double)ypart;
-      }
-    }
-#pragma unroll
-    for (int gg = 0; gg < NG; ++gg) {
-      ah[gg] = nh[gg];
-      al[gg] = nl[gg];
-    }
-#pragma unroll
-    for (int i = 0; i < NR; ++i) r0v[i] = nr0[i];
-  }
+    // wavefront-uniform tile range (scalar loop control)
+    const int u0 = __builtin_amdgcn_readfirstlane(t0), u1 = __builtin_amdgcn_readfirstlane(t1);
+    if (G == 4)
+      lr_bf16_tiles<LPC, NJ, 4, STATS>(P, Bs, u0, u1, t4, ss, sy);
+    else if (G == 2)
+      lr_bf16_tiles<LPC, NJ, 2, STATS>(P, Bs, u0, u1, t4, ss, sy);
+    else if (G == 1)
+      lr_bf16_tiles<LPC, NJ, 1, STATS>(P, Bs, u0, u1, t4, ss, sy);
+    else
+      lr_bf16_tiles<LPC, NJ, 3, STATS>(P, Bs, u0, u1, t4, ss, sy);
   }
   // partial sums of every group from this wavefront's rows -> LDS [group][wavefront][chain]
 #pragma unroll
