@@ -259,22 +259,46 @@ def instrumented_slice_timing(wl, R, lpc, wpb, seed0):
     return np.array(durs), np.array(work), np.array(evals)
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(wl, spec, gpu_evals_per_run, budget_s=12.0):
     """The oracle (a C port of the reference algorithm) on the host cores.  A shell-limited probe
     measures evaluations/s; when a whole run is predicted to fit twice in `budget_s`, whole runs of
     the same workload (different seeds) are timed and runs/s is reported directly, otherwise the
     probe is grown to about a third of the budget and extrapolated with the device's evaluations
-    per run."""
+    per run.  A shell-limited sample on one thread gives the single-core rate beside it."""
     import oracle as O
     ospec = wl.oracle_spec(spec)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
-    def run(seed, iters):
+    def run(seed, iters, nthreads=threads):
         t0 = time.perf_counter()
         r = O.ns_run(ospec, num_live_points=spec.num_live_points, num_slices=spec.num_slices,
-                     max_samples=spec.max_samples, seed=seed, num_threads=threads, max_iterations=iters)
+                     max_samples=spec.max_samples, seed=seed, num_threads=nthreads, max_iterations=iters)
         return time.perf_counter() - t0, r["total_likelihood_evaluations"]
 
+    def one_core():
+        it = 1
+        while True:
+            dt1, ev1 = run(5, it, 1)
+            if dt1 > 1.5 or it >= 256:
+                break
+            it *= 2
+        print(f"[bench] cpu baseline, one thread: {it} shells in {dt1:.2f} s", file=sys.stderr, flush=True)
+        return dict(evals_per_s=ev1 / dt1, runs_per_s=ev1 / dt1 / gpu_evals_per_run,
+                    sample=f"{it} shells of one run on one thread, {ev1} likelihood evaluations in {dt1:.2f} s, "
+                           f"runs/s extrapolated with {gpu_evals_per_run:.3g} evaluations per full run")
+
+    info = dict(cpu_model=cpu_model(), nproc=os.cpu_count())
     iters = 2
     while True:
         dt, ev = run(5, iters)
@@ -295,7 +319,7 @@ def cpu_baseline(wl, spec, gpu_evals_per_run, budget_s=12.0):
                     sample=f"{runs} whole runs of the same workload (create_init_state, shells until the "
                            f"termination condition, to_results; D={spec.model.U_ndims}, num_live_points="
                            f"{spec.num_live_points}, different seeds), {evals} likelihood evaluations in "
-                           f"{t_all:.2f} s")
+                           f"{t_all:.2f} s", one_core=one_core(), **info)
     while dt < budget_s / 3 and iters < 4096:
         iters *= 2
         dt, ev = run(5, iters)
@@ -305,7 +329,7 @@ def cpu_baseline(wl, spec, gpu_evals_per_run, budget_s=12.0):
                 sample=f"{iters} shells of one run of the same workload (create_init_state + {iters} x "
                        f"_collect_shell + to_results; D={spec.model.U_ndims}, num_live_points="
                        f"{spec.num_live_points}), {ev} likelihood evaluations in {dt:.2f} s, extrapolated "
-                       f"with {gpu_evals_per_run:.3g} evaluations per full run")
+                       f"with {gpu_evals_per_run:.3g} evaluations per full run", one_core=one_core(), **info)
 
 
 def log(*a):
