@@ -347,13 +347,15 @@ def precision_at_scale(wl, R, shells, lpc, wpb, steps):
     for terms, label in ((3, "bf16x3"), (1, "fp32")):
         specs = [RunSpec(model=wl.model, num_live_points=wl.N, num_slices=wl.D * wl.s, k_phantom=0,
                          max_samples=wl.N * (shells + 4), seed=sd) for sd in range(R)]
-        b = DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=wpb, max_steps=512, lr_terms=terms)
+        refine = terms != 1
+        b = DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=wpb, max_steps=512, lr_terms=terms,
+                        refine_fp32=refine)
         b.init_state()
         tc = make_term_cond(max_samples=float(specs[0].max_samples))
         s = stream_ptr()
-        durs, evs = [], []
+        durs, evs, rdurs = [], [], []
         prev = sum(st.num_likelihood_evaluations for st in b.states())
-        errs = []
+        errs, raw_errs = [], []
         for k in range(shells):
             check(lib().hipns_shell_discard(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host, C.byref(b.bufs), s))
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -369,6 +371,16 @@ def precision_at_scale(wl, R, shells, lpc, wpb, steps):
             sel = np.linspace(0, n0 - 1, 64).astype(int)
             from scipy.special import ndtri
             Lh = wl.lik.log_prob_numpy(wl.tau * ndtri(U[sel]))
+            if refine:
+                # the accepted points re-evaluated in plain FP32 before they enter the live set
+                raw_errs.append(np.abs(Ld[sel] - Lh) / np.abs(Lh))
+                r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                r0.record()
+                b.refine_new()
+                r1.record()
+                torch.cuda.synchronize()
+                rdurs.append(r0.elapsed_time(r1) * 1e-3)
+                Ld = b.new_logL[:n0].cpu().numpy()
             errs.append(np.abs(Ld[sel] - Lh) / np.abs(Lh))
             check(lib().hipns_shell_replace(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host, C.byref(b.bufs), s))
             check(lib().hipns_shell_register(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host,
@@ -385,6 +397,11 @@ def precision_at_scale(wl, R, shells, lpc, wpb, steps):
         out[label] = dict(evals_per_s=eps, avg_launch_ms=float(durs.mean() * 1e3), evals_per_launch=float(evs.mean()),
                           logL_rel_err_max=float(e.max()), logL_rel_err_median=float(np.median(e)),
                           achieved_TFLOPs=eps * wl.work_per_eval(None) / 1e12)
+        if refine:
+            er = np.concatenate(raw_errs)
+            out[label].update(refined_fp32=True, refine_ms_per_shell=float(np.mean(rdurs) * 1e3),
+                              unrefined_logL_rel_err_max=float(er.max()),
+                              unrefined_logL_rel_err_median=float(np.median(er)))
         del b
     return out
 

@@ -97,7 +97,8 @@ class DeviceBatch:
     """R independent runs sharing one likelihood family, advanced together."""
 
     def __init__(self, specs: List[RunSpec], *, lanes_per_chain: int = 2, waves_per_block: int = 4,
-                 max_steps: int = 512, device: str = "cuda", lr_terms: int = 3, gradient_guided: bool = False):
+                 max_steps: int = 512, device: str = "cuda", lr_terms: int = 3, gradient_guided: bool = False,
+                 refine_fp32: bool = False):
         torch = _torch()
         if not specs:
             raise ValueError("no runs")
@@ -214,6 +215,15 @@ class DeviceBatch:
             raise ValueError("lr_terms must be 1 (plain FP32), 3 (BF16x3) or 4")
         self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, lr_ndata, max_steps, lr_panels,
                              lr_terms, lr_off)
+        # FP32 refinement of accepted points (regression family in a BF16 form): the BF16 product decides
+        # acceptance, the plain FP32 form re-evaluates each shell's accepted points before they enter
+        # the live set, so the stored logL (what the evidence integrates) has FP32 error
+        self.refine_fp32 = bool(refine_fp32)
+        if self.refine_fp32:
+            if self.external or self.family != _lib.FAM_LINREG or lr_terms == 1:
+                raise ValueError("refine_fp32 applies to the regression family in a BF16 form (lr_terms 3 or 4)")
+            self.launch_fp32 = Launch.from_buffer_copy(self.launch)
+            self.launch_fp32.lr_terms = 1
         if self.external:
             i32 = torch.int32
             self.ext_U0 = torch.full((sumN, dps), 0.5, dtype=f32, device=dev)
@@ -286,8 +296,31 @@ class DeviceBatch:
                                             C.byref(tc), int(set_no_seed), self._s(stream)),
               "hipns_check_termination")
 
+    def refine_new(self, stream=None):
+        """Re-evaluate the accepted points of the current shell (every run's new samples) with the plain
+        FP32 regression likelihood and overwrite their logL (hipns_log_likelihood, one launch per run).
+        Phantom samples keep the BF16 value; evaluation counts do not include the refinement."""
+        s = self._s(stream)
+        sz = C.sizeof(RunDesc)
+        base_U, base_L, base_runs = ptr(self.new_U), ptr(self.new_logL), ptr(self.runs_dev)
+        for r in range(self.R):
+            d = self.runs_host[r]
+            n, off = int(d.shell), int(d.chain_off)
+            check(lib().hipns_log_likelihood(base_U + off * self.dps * 4, n, base_L + off * 8, base_runs + r * sz,
+                                             C.pointer(self.runs_host[r]), C.byref(self.bufs),
+                                             C.byref(self.launch_fp32), s), "hipns_log_likelihood")
+
     def collect_shell(self, term: dict, stream=None):
         tc = _lib.make_term_cond(**term)
+        if self.refine_fp32:
+            s = self._s(stream)
+            args = (ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host, C.byref(self.bufs))
+            check(lib().hipns_shell_discard(*args, s), "hipns_shell_discard")
+            check(lib().hipns_shell_sample(*args, C.byref(self.launch), s), "hipns_shell_sample")
+            self.refine_new(stream)
+            check(lib().hipns_shell_replace(*args, s), "hipns_shell_replace")
+            check(lib().hipns_shell_register(*args, C.byref(tc), s), "hipns_shell_register")
+            return
         if self.external:
             s = self._s(stream)
             args = (ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host, C.byref(self.bufs))

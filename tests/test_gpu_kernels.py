@@ -361,6 +361,52 @@ def test_batched_runs_match_single_runs():
     assert a.log_Z_mean == b.log_Z_mean
 
 
+def test_linreg_fp32_refinement():
+    """FP32 refinement of accepted points (regression family in BF16x3): after a shell, refine_new
+    overwrites every run's accepted-point logL with the plain FP32 form, which agrees with the FP64
+    likelihood at the same points to 1e-6 relative (the BF16x3 values themselves to 3e-5); a full run
+    with refinement gives log Z within the stated uncertainty of the analytic evidence."""
+    from scipy.special import ndtri
+    from jaxns_amd._lib import make_term_cond
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    torch = _torch()
+    say("linreg fp32 refinement")
+    model, log_Z, lik = linreg_problem(2048, 16, seed=5, sigma=0.3, tau=1.0)
+    specs = [RunSpec(model=model, num_live_points=256, num_slices=5 * 16, k_phantom=0, max_samples=256 * 300,
+                     seed=sd) for sd in (1, 2)]
+    b = DeviceBatch(specs, lanes_per_chain=2, waves_per_block=4, lr_terms=3, refine_fp32=True)
+    b.init_state()
+    s = _stream()
+    args = (ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host, C.byref(b.bufs))
+    tc = make_term_cond(max_samples=1e18)
+    for shell in range(3):
+        check(lib().hipns_shell_discard(*args, s))
+        check(lib().hipns_shell_sample(*args, C.byref(b.launch), s))
+        torch.cuda.synchronize()
+        raw = b.new_logL.clone()
+        b.refine_new()
+        torch.cuda.synchronize()
+        for r in range(b.R):
+            d = b.runs_host[r]
+            off, n = int(d.chain_off), int(d.shell)
+            U = b.new_U[off:off + n, :16].double().cpu().numpy()
+            ref = lik.log_prob_numpy(1.0 * ndtri(U))
+            got = b.new_logL[off:off + n].cpu().numpy()
+            bf = raw[off:off + n].cpu().numpy()
+            rel = np.abs(got - ref) / np.abs(ref)
+            say("shell", shell, "run", r, "refined max rel", rel.max(), "bf16x3 max rel", (np.abs(bf - ref) / np.abs(ref)).max())
+            assert rel.max() < 1e-6
+            assert (np.abs(bf - ref) / np.abs(ref)).max() < 3e-5
+        check(lib().hipns_shell_replace(*args, s))
+        check(lib().hipns_shell_register(*args, C.byref(tc), s))
+    torch.cuda.synchronize()
+    full = DeviceBatch(specs[:1], lanes_per_chain=2, waves_per_block=4, lr_terms=3, refine_fp32=True)
+    full.run(dict(dlogZ=math.log(1.0 + 1e-3), max_samples=1e18), graph_iters=4)
+    z = full.log_Z()
+    say("refined run log Z", z[0], "analytic", log_Z)
+    assert abs(z[0, 0] - log_Z) < 4 * z[0, 1] + 0.2
+
+
 def test_uneven_batched_gmm_runs_logZ():
     """Runs of uneven dimension and live-point count (mixture likelihood, uniform box prior)
     advanced in one batch: each run's log Z against its own analytic value."""
