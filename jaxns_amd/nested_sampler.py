@@ -34,14 +34,20 @@ class NestedSamplerState:
 
 
 class NestedSampler:
-    """public.py:31-222 -- static nested sampling with the uni-dimensional slice sampler."""
+    """public.py:31-222 -- static nested sampling with the uni-dimensional slice sampler.
+
+    The reference's arguments keep their names, meanings and defaults.  Keyword-only extras:
+    lanes_per_chain / waves_per_block / max_steps shape the sampler launch; lr_terms picks the
+    regression family's product form (3 = BF16x3, 4 = BF16x3 plus the third weight part, 1 = plain
+    FP32); refine_fp32 re-evaluates each shell's accepted points in plain FP32 before they enter the
+    live set (regression family in a BF16 form)."""
 
     def __init__(self, model: Model, max_samples: Optional[float] = None, num_live_points: Optional[int] = None,
                  num_slices: Optional[int] = None, s: Optional[float] = None, k: Optional[int] = None,
                  c: Optional[int] = None, devices=None, difficult_model: bool = False,
                  parameter_estimation: bool = False, shell_fraction: float = 0.5, gradient_guided: bool = False,
                  init_efficiency_threshold: float = 0.1, verbose: bool = False, *, lanes_per_chain: int = 2,
-                 waves_per_block: int = 4, max_steps: int = 512, lr_terms: int = 3):
+                 waves_per_block: int = 4, max_steps: int = 512, lr_terms: int = 3, refine_fp32: bool = False):
         self.model = model
         D = model.U_ndims
         # public.py:69-107
@@ -85,7 +91,7 @@ class NestedSampler:
         self.init_efficiency_threshold = init_efficiency_threshold
         self.verbose = verbose
         self.launch_kw = dict(lanes_per_chain=lanes_per_chain, waves_per_block=waves_per_block, max_steps=max_steps,
-                              lr_terms=lr_terms, gradient_guided=self.gradient_guided)
+                              lr_terms=lr_terms, gradient_guided=self.gradient_guided, refine_fp32=refine_fp32)
         probe = self._spec(0)
         self.num_live_points = probe.num_live_points
         self._max_samples_rounded = probe.max_samples
