@@ -39,9 +39,9 @@ PEAK_FP32_MFMA = 157.3e12
 
 # HBM bytes per k_sample launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the same
 # instrumented batch (counters scaled by a 256 MiB copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
-# profiles/k_sample_mvn_d32_n2048_r128.md and profiles/k_sample_linreg_4096x16_n1024_r256.md.
+# profiles/k_sample_mvn_d32_n2048_r512.md and profiles/k_sample_linreg_4096x16_n1024_r256.md.
 # Keyed by (workload, runs per launch); other shapes report null.
-MEASURED_TRAFFIC = {("c2", 128): 46.47e6, ("c4", 256): 32.51e6}
+MEASURED_TRAFFIC = {("c2", 512): 184.93e6, ("c4", 256): 32.51e6}
 PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
 
@@ -411,7 +411,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--runs", type=int, default=None, help="runs per step (chosen form); default 128, c3/c4: 256")
+    ap.add_argument("--runs", type=int, default=None,
+                    help="runs per step (chosen form); default c2: 512, c1: 128, c3/c4: 256, c5: 8")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--shells", type=int, default=3, help="c5: shells per partial run")
     ap.add_argument("--graph-iters", type=int, default=8)
@@ -435,7 +436,9 @@ def main():
     wl = make_workload(args.config)
     if args.config == "c4":
         wl.lr_terms = args.lr_terms
-    R = args.runs or (256 if args.config in ("c3", "c4") else (8 if args.config == "c5" else 128))
+    # c2: 512 runs per step (128: 793 runs/s, 256: 867, 512: 899, 1024: 921; the shell kernel's tail and
+    # the per-run kernels amortise over more runs)
+    R = args.runs or {"c2": 512, "c3": 256, "c4": 256, "c5": 8}.get(args.config, 128)
     seed_base = 0
     if args.config == "c5":
         t0 = time.perf_counter()
