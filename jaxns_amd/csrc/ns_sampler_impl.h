@@ -520,30 +520,35 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
   prior_q<NJ>(up, qv, tfm);
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    if (j < nj) {
-      int d = owned_dim<LPC>(h, j);
-      if (d < D) {
-        float x = fmaf(P.q[d], qv[j], P.p[d]);
+    if (j < nj) {  // uniform per run
+      const int d = owned_dim<LPC>(h, j);
+      const bool in = d < D;  // padding slots contribute nothing
+      const float x = fmaf(P.q[d], qv[j], P.p[d]);
 #pragma unroll
-        for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
-          if (k < P.K) {
-            float r = x - P.mean[k * P.dps + d];
-            part[k] = fmaf(r, r, part[k]);
-          }
+      for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
+        if (k < P.K) {
+          const float r = in ? x - P.mean[k * P.dps + d] : 0.0f;
+          part[k] = fmaf(r, r, part[k]);
         }
       }
     }
   }
-  double acc = -__builtin_inf();
+  // log sum_k exp(logc_k - |x - m_k|^2 / (2 s_k^2)) with the maximum taken out, in FP32 with the
+  // hardware exp2 / log2 (the component terms are FP32 already), widened to FP64 at the end
+  float mx = -__builtin_inff();
 #pragma unroll
   for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
     if (k < P.K) {
-      float s = chain_sum<LPC>(part[k]);
-      double lk = (double)P.logc[k] - (double)s * (double)P.inv2s2[k];
-      acc = lae(acc, lk);
+      part[k] = fmaf(-chain_sum<LPC>(part[k]), P.inv2s2[k], P.logc[k]);  // component log-density
+      mx = fmaxf(mx, part[k]);
     }
   }
-  return __builtin_isnan(acc) ? -__builtin_inf() : acc;
+  float sum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < HIPNS_MAX_GMM_K; ++k)
+    if (k < P.K) sum += __builtin_amdgcn_exp2f((part[k] - mx) * 1.44269504088896341f);
+  const double L = (double)mx + (double)(0.693147180559945309f * __builtin_amdgcn_logf(sum));
+  return __builtin_isnan(L) ? -__builtin_inf() : L;
 }
 
 // bit j set when owned dimension j of lane-part h uses q = ndtri
