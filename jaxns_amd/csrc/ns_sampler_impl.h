@@ -183,9 +183,9 @@ __device__ __forceinline__ typename LrAcc<LPC>::type lr_mfma(const bf16x8_t& a, 
 
 // The BF16 residual tiles [t0, t1) of one wavefront against the B operands of GG chain groups (GG
 // known at compile time, so the groups' MFMA chains are interleaved and nothing branches per group).
-// Tiles are double-buffered with wavefront-uniform bounds: the next tile's X panels and r0 load while
-// this one multiplies.  With few (group, k-group) pairs the step's B operands are held in registers
-// for all tiles; otherwise they are read from LDS per tile.
+// The tile loop has wavefront-uniform bounds; at LPC=4 the tiles are double-buffered (the next tile's
+// X panels and r0 load while this one multiplies).  With few (group, k-group) pairs the step's B
+// operands are held in registers for all tiles; otherwise they are read from LDS per tile.
 template <int LPC, int NJ, int GG, bool STATS>
 __device__ __forceinline__ void lr_bf16_tiles(const LinRegLds& P, const bf16x8_t* Bs, int t0, int t1, bool t4,
                                               double (&ss)[LR_MAX_G], double (&sy)[LR_MAX_G]) {
@@ -273,15 +273,27 @@ __device__ __forceinline__ void lr_bf16_tiles(const LinRegLds& P, const bf16x8_t
       }
     }
   };
-  bf16x8_t xh0[NG], xl0[NG], xh1[NG], xl1[NG];
-  float r00[NR], r01[NR];
-  if (t0 < t1) load(t0, xh0, xl0, r00);
-  for (int tl = t0; tl < t1; tl += 2) {
-    if (tl + 1 < t1) load(tl + 1, xh1, xl1, r01);
-    compute(tl, xh0, xl0, r00);
-    if (tl + 1 >= t1) break;
-    if (tl + 2 < t1) load(tl + 2, xh0, xl0, r00);
-    compute(tl + 1, xh1, xl1, r01);
+  bf16x8_t xh0[NG], xl0[NG];
+  float r00[NR];
+  if constexpr (LPC == 4) {
+    // double-buffered: the next tile's X panels and r0 load while this one multiplies
+    bf16x8_t xh1[NG], xl1[NG];
+    float r01[NR];
+    if (t0 < t1) load(t0, xh0, xl0, r00);
+    for (int tl = t0; tl < t1; tl += 2) {
+      if (tl + 1 < t1) load(tl + 1, xh1, xl1, r01);
+      compute(tl, xh0, xl0, r00);
+      if (tl + 1 >= t1) break;
+      if (tl + 2 < t1) load(tl + 2, xh0, xl0, r00);
+      compute(tl + 1, xh1, xl1, r01);
+    }
+  } else {
+    // LPC=2: the second buffer (24 VGPRs) would drop the kernel to one wavefront per SIMD, which
+    // costs more than the exposed load latency (4096 x 16, 128 runs: 17.1 ms with it, 14.6 without)
+    for (int tl = t0; tl < t1; ++tl) {
+      load(tl, xh0, xl0, r00);
+      compute(tl, xh0, xl0, r00);
+    }
   }
 }
 
