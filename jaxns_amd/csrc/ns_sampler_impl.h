@@ -545,21 +545,29 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
       }
     }
   }
-  // log sum_k exp(logc_k - |x - m_k|^2 / (2 s_k^2)) with the maximum taken out, in FP32 with the
-  // hardware exp2 / log2 (the component terms are FP32 already), widened to FP64 at the end
-  float mx = -__builtin_inff();
+  // log sum_k exp(c_k), c_k = logc_k - |x - m_k|^2 / (2 s_k^2), as c_top + log sum_k exp(c_k - c_top)
+  // about the largest component: c_top is assembled in FP64 from its FP32 square sum (so points near
+  // a peak keep distinct values), the correction (>= 0, <= log K) in FP32 with the hardware exp2 / log2
+  float mx = -__builtin_inff(), ss_top = 0.0f, i_top = 0.0f, c_top = 0.0f;
 #pragma unroll
   for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
     if (k < P.K) {
-      part[k] = fmaf(-chain_sum<LPC>(part[k]), P.inv2s2[k], P.logc[k]);  // component log-density
-      mx = fmaxf(mx, part[k]);
+      const float ss = chain_sum<LPC>(part[k]);
+      part[k] = fmaf(-ss, P.inv2s2[k], P.logc[k]);  // component log-density, FP32
+      if (!(part[k] <= mx)) {  // first maximum (a NaN component is taken, and propagates)
+        mx = part[k];
+        ss_top = ss;
+        i_top = P.inv2s2[k];
+        c_top = P.logc[k];
+      }
     }
   }
   float sum = 0.0f;
 #pragma unroll
   for (int k = 0; k < HIPNS_MAX_GMM_K; ++k)
     if (k < P.K) sum += __builtin_amdgcn_exp2f((part[k] - mx) * 1.44269504088896341f);
-  const double L = (double)mx + (double)(0.693147180559945309f * __builtin_amdgcn_logf(sum));
+  const double L = fma(-(double)ss_top, (double)i_top, (double)c_top) +
+                   (double)(0.693147180559945309f * __builtin_amdgcn_logf(sum));
   return __builtin_isnan(L) ? -__builtin_inf() : L;
 }
 
