@@ -406,6 +406,61 @@ def precision_at_scale(wl, R, shells, lpc, wpb, steps):
     return out
 
 
+def to_results_scale(M, iters=5):
+    """The post-path primitives of _to_results at M samples, each timed with HIP events on the stream it
+    runs on: the evidence scan (compute_evidence_stats, 80 algorithmic bytes per sample: read log L and n,
+    write the 8-field EvidenceCalculation), count_crossed_edges (28 B per node: read sender and log L,
+    write the sample index and the live-point count) and the stable FP64 argsort (16 B per key)."""
+    import torch
+    from jaxns_amd._lib import check, lib, ptr, stream_ptr
+    L = lib()
+    g = torch.Generator().manual_seed(11)
+    keys = torch.randn(M, dtype=torch.float64, generator=g).cuda()
+    logL_sorted = torch.sort(keys)[0]
+    # a static run's live-point counts: N throughout, then N..1 over the last N samples
+    N = 4096
+    n = torch.full((M,), float(N), dtype=torch.float64, device="cuda")
+    n[M - N:] = torch.arange(N, 0, -1, dtype=torch.float64, device="cuda")
+    # sender of sample i: a uniform earlier node (0 is the root)
+    sender = (torch.rand(M, generator=g, dtype=torch.float64) * torch.arange(1, M + 1, dtype=torch.float64))
+    sender = sender.floor().to(torch.int32).cuda()
+    per = torch.empty((8, M), dtype=torch.float64, device="cuda")
+    fin = torch.empty(8, dtype=torch.float64, device="cuda")
+    si = torch.empty(M, dtype=torch.int64, device="cuda")
+    nl = torch.empty(M, dtype=torch.int64, device="cuda")
+    perm = torch.empty(M, dtype=torch.int64, device="cuda")
+    ws_e = torch.empty(int(L.hipns_evidence_scan_workspace_size(M)), dtype=torch.uint8, device="cuda")
+    ws_c = torch.empty(int(L.hipns_count_crossed_edges_workspace_size(M)), dtype=torch.uint8, device="cuda")
+    ws_a = torch.empty(int(L.hipns_argsort_workspace_size(M)), dtype=torch.uint8, device="cuda")
+    s = stream_ptr()
+    calls = {
+        "evidence_scan": (80, lambda: check(L.hipns_evidence_scan(ptr(logL_sorted), ptr(n), M, -1, ptr(per), ptr(fin),
+                                                                  ptr(ws_e), ws_e.numel(), s), "hipns_evidence_scan")),
+        "count_crossed_edges": (28, lambda: check(L.hipns_count_crossed_edges(ptr(sender), ptr(keys), M, -1, ptr(si),
+                                                                              ptr(nl), ptr(ws_c), ws_c.numel(), s),
+                                                  "hipns_count_crossed_edges")),
+        "argsort_stable": (16, lambda: check(L.hipns_argsort_stable(ptr(keys), M, ptr(perm), ptr(ws_a), ws_a.numel(), s),
+                                             "hipns_argsort_stable")),
+    }
+    out = {}
+    for name, (bps, fn) in calls.items():
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        gbs = bps * M / (ms * 1e-3) / 1e9
+        out[name] = dict(avg_ms=ms, algorithmic_bytes=bps * M, achieved_GBps=gbs, frac=gbs * 1e9 / PEAK_HBM)
+        log(name, f"{ms:.3f} ms, {gbs:.0f} GB/s")
+    # the argsort agrees with torch's stable sort (the sort of count_crossed_edges is the same code)
+    ok = bool(torch.equal(perm, torch.sort(keys, stable=True)[1]))
+    return dict(to_results_scale=True, M=M, iters=iters, kernels=out, argsort_matches_torch=ok)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -424,6 +479,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="use --lpc/--wpb instead of the autotuner")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only the instrumented shell-by-shell pass (for rocprofv3 kernel statistics)")
+    ap.add_argument("--to-results-scale", type=int, default=0,
+                    help="time the evidence scan, count_crossed_edges and the stable argsort at this many samples")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="with --profile-pass: also copy a buffer of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -433,6 +490,9 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.to_results_scale:
+        print(json.dumps(to_results_scale(args.to_results_scale, args.steps)), flush=True)
+        return
     wl = make_workload(args.config)
     if args.config == "c4":
         wl.lr_terms = args.lr_terms
