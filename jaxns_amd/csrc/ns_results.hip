@@ -229,115 +229,302 @@ hipError_t evidence_scan(const double* logL, const double* n, int64_t M, int64_t
 }
 
 // ===========================================================================
-// Stable argsort of FP64 keys: bitonic sort of 2048-element tiles in LDS by
-// (key, index), then merge passes where each element finds its output slot by
-// binary search in the partner run (ties: left run first).
+// Stable argsort of FP64 keys (jnp.argsort(stable=True) order: -inf < ... < +inf < NaN, -0 == +0,
+// equal keys keep their input order) as an LSD radix sort of order-preserving 64-bit codes: eight
+// passes over 8-bit digits.  Each pass is three launches: per-tile digit histograms, a scan of every
+// (segment, digit) row of tile counts, and a stable scatter that ranks each tile's elements by digit
+// in input order, stages the tile in LDS in digit order and writes each digit's run contiguously.
+// A launch may sort R segments of a common stride S at once (grid.y = segment), each of its own
+// length; nothing waits on another workgroup.
 // ===========================================================================
-constexpr int SORT_THREADS = 256;
-constexpr int SORT_TILE = 2048;
+constexpr int RS_THREADS = 256;  // == RS_BINS: thread d owns digit d in the per-digit steps
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_BINS = 256;
+constexpr int RS_WAVES = RS_THREADS / 64;
 
-__global__ void __launch_bounds__(SORT_THREADS) k_tile_sort(const double* keys_in, int64_t n, double* keys_out,
-                                                            int32_t* idx_out) {
-  __shared__ double k[SORT_TILE];
-  __shared__ int32_t v[SORT_TILE];
-  const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-  for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
-    int64_t g = base + i;
-    k[i] = g < n ? keys_in[g] : __builtin_nan("");
-    v[i] = g < n ? (int32_t)g : (int32_t)0x7fffffff;
+// order-preserving code: negative -> bit complement, non-negative -> sign bit set; -0 as +0; every NaN
+// one code above +inf.  Codes are equal exactly when the keys compare equal (or are both NaN).
+__device__ __forceinline__ uint64_t sort_code(double x) {
+  if (__builtin_isnan(x)) return ~0ull;
+  if (x == 0.0) x = 0.0;
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+template <bool FIRST>
+__device__ __forceinline__ uint64_t rs_key(const void* kin, int64_t g) {
+  if constexpr (FIRST) return sort_code(((const double*)kin)[g]);
+  else return ((const uint64_t*)kin)[g];
+}
+
+// lanes of the wavefront that are valid and hold the same digit as this lane
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// exclusive scan of one value per thread over a 256-thread block; *total gets the sum (all threads)
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wsum, uint32_t* total) {
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < RS_WAVES; ++q) {
+    if (q < w) pre += s_wsum[q];
+    tot += s_wsum[q];
   }
   __syncthreads();
-  for (int kk = 2; kk <= SORT_TILE; kk <<= 1) {
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
-        int p = i ^ j;
-        if (p > i) {
-          bool up = (i & kk) == 0;
-          double ki = k[i], kp = k[p];
-          int32_t ii = v[i], ip = v[p];
-          bool sw = up ? key_lt(kp, ip, ki, ii) : key_lt(ki, ii, kp, ip);
-          if (sw) {
-            k[i] = kp;
-            k[p] = ki;
-            v[i] = ip;
-            v[p] = ii;
-          }
-        }
-      }
-      __syncthreads();
+  *total = tot;
+  return pre + x - v;
+}
+
+// segment lengths: one array of n keys, or run r's nodes (M_r + 1) of a batch
+struct SegFixed {
+  int64_t n;
+  __device__ int64_t len(int) const { return n; }
+};
+
+__device__ __forceinline__ int64_t batch_M(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int r) {
+  const int64_t ns = states[r].num_samples, ms = runs[r].max_samples;
+  return ns < ms ? ns : ms;
+}
+
+struct SegBatchNodes {
+  const hipns_run_desc_t* runs;
+  const hipns_run_state_t* states;
+  __device__ int64_t len(int r) const { return batch_M(runs, states, r) + 1; }
+};
+
+// counts[(seg * 256 + digit) * T + tile] = elements of the tile with that digit
+template <bool FIRST, class Seg>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_count(const void* kin, int64_t S, int T, Seg seg, int shift,
+                                                         uint32_t* counts) {
+  __shared__ uint32_t hist[RS_BINS];
+  const int r = blockIdx.y, t = blockIdx.x, tid = threadIdx.x;
+  const int64_t n = seg.len(r);
+  hist[tid] = 0u;
+  __syncthreads();
+  const int64_t base = (int64_t)t * RS_TILE;
+  if (base < n) {  // block-uniform
+    const uint64_t lt = (1ull << lane_id()) - 1ull;
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) {
+      const int64_t i = base + k * RS_THREADS + tid;
+      const bool valid = i < n;
+      const uint32_t d = valid ? (uint32_t)(rs_key<FIRST>(kin, (int64_t)r * S + i) >> shift) & 255u : 0u;
+      const uint64_t peers = digit_peers(d, valid);
+      if (valid && (peers & lt) == 0ull) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     }
   }
-  for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
-    int64_t g = base + i;
-    if (g < n) {
-      keys_out[g] = k[i];
-      idx_out[g] = v[i];
+  __syncthreads();
+  counts[((int64_t)r * RS_BINS + tid) * T + t] = hist[tid];
+}
+
+// each (digit, segment) row of T tile counts -> exclusive offsets in place; totals[seg * 256 + digit]
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scan(uint32_t* counts, int T, uint32_t* totals) {
+  __shared__ uint32_t s_wsum[RS_WAVES];
+  const int d = blockIdx.x, r = blockIdx.y;
+  uint32_t* c = counts + ((int64_t)r * RS_BINS + d) * T;
+  const int per = (T + RS_THREADS - 1) / RS_THREADS;
+  const int j0 = threadIdx.x * per;
+  const int j1 = j0 + per < T ? j0 + per : T;
+  uint32_t sum = 0u;
+  for (int j = j0; j < j1; ++j) sum += c[j];
+  uint32_t tot;
+  uint32_t run = block_excl_scan(sum, s_wsum, &tot);
+  for (int j = j0; j < j1; ++j) {
+    const uint32_t v = c[j];
+    c[j] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) totals[r * RS_BINS + d] = tot;
+}
+
+// stable scatter of one pass; the first pass reads FP64 keys (codes formed here) and takes the
+// element's position as its value, the last writes values only
+template <bool FIRST, bool LAST, class Seg>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const void* kin, const int32_t* vin, int64_t S, int T,
+                                                           Seg seg, int shift, const uint32_t* offs,
+                                                           const uint32_t* totals, uint64_t* kout, int32_t* vout) {
+  __shared__ uint64_t s_k[RS_TILE];
+  __shared__ int32_t s_v[RS_TILE];
+  __shared__ uint32_t s_wcnt[2][RS_WAVES][RS_BINS];
+  __shared__ uint32_t s_run[RS_BINS];
+  __shared__ uint32_t s_toff[RS_BINS];
+  __shared__ uint32_t s_gbase[RS_BINS];
+  __shared__ uint32_t s_wsum[RS_WAVES];
+  const int r = blockIdx.y, t = blockIdx.x, tid = threadIdx.x;
+  const int64_t n = seg.len(r);
+  const int64_t base = (int64_t)t * RS_TILE;
+  if (base >= n) return;  // block-uniform: an empty tile places nothing
+  const int w = tid >> 6;
+  const uint64_t lt = (1ull << lane_id()) - 1ull;
+  const int64_t so = (int64_t)r * S;
+  {
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(totals[r * RS_BINS + tid], s_wsum, &tot);
+    s_gbase[tid] = ex + offs[((int64_t)r * RS_BINS + tid) * T + t];
+    s_run[tid] = 0u;
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; ++q) {
+      s_wcnt[0][q][tid] = 0u;
+      s_wcnt[1][q][tid] = 0u;
     }
+  }
+  __syncthreads();
+  uint64_t kr[RS_ITEMS];
+  int32_t vr[RS_ITEMS];
+  uint32_t dr[RS_ITEMS];  // digit | (rank among the tile's elements of that digit) << 8; ~0 = none
+#pragma unroll
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    const int buf = k & 1;
+    const int64_t i = base + k * RS_THREADS + tid;
+    const bool valid = i < n;
+    const uint64_t key = valid ? rs_key<FIRST>(kin, so + i) : ~0ull;
+    int32_t val = 0;
+    if (valid) val = FIRST ? (int32_t)i : vin[so + i];
+    const uint32_t d = (uint32_t)(key >> shift) & 255u;
+    const uint64_t peers = digit_peers(d, valid);
+    const uint32_t rw = (uint32_t)__popcll(peers & lt);
+    if (valid && rw == 0u) s_wcnt[buf][w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    uint32_t pre = s_run[d];
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; ++q)
+      if (q < w) pre += s_wcnt[buf][q][d];
+    kr[k] = key;
+    vr[k] = val;
+    dr[k] = valid ? (d | ((pre + rw) << 8)) : ~0u;
+    __syncthreads();
+    // thread tid folds digit tid's counts of this round and clears them for the round after next
+    uint32_t add = 0u;
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; ++q) {
+      add += s_wcnt[buf][q][tid];
+      s_wcnt[buf][q][tid] = 0u;
+    }
+    s_run[tid] += add;
+  }
+  __syncthreads();
+  {
+    uint32_t tot;
+    s_toff[tid] = block_excl_scan(s_run[tid], s_wsum, &tot);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    if (dr[k] != ~0u) {
+      const uint32_t pos = s_toff[dr[k] & 255u] + (dr[k] >> 8);
+      s_k[pos] = kr[k];
+      s_v[pos] = vr[k];
+    }
+  }
+  __syncthreads();
+  const int cnt = (int)(n - base < RS_TILE ? n - base : RS_TILE);
+  for (int j = tid; j < cnt; j += RS_THREADS) {
+    const uint64_t key = s_k[j];
+    const uint32_t d = (uint32_t)(key >> shift) & 255u;
+    const int64_t o = so + s_gbase[d] + (uint32_t)j - s_toff[d];
+    if constexpr (!LAST) kout[o] = key;
+    vout[o] = s_v[j];
   }
 }
 
-// merge runs of length `run` pairwise: element of the left run goes before equal keys of the right run
-__global__ void k_merge_pass(const double* kin, const int32_t* vin, int64_t n, int64_t run, double* kout,
-                             int32_t* vout) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  int64_t pair = g / (2 * run);
-  int64_t lbeg = pair * 2 * run;
-  int64_t rbeg = lbeg + run;
-  int64_t lend = rbeg < n ? rbeg : n;
-  int64_t rend = rbeg + run < n ? rbeg + run : n;
-  double key = kin[g];
-  int64_t pos;
-  if (g < rbeg) {
-    // left element: count right elements strictly less
-    int64_t lo = rbeg, hi = rend;
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if (kless(kin[mid], key)) lo = mid + 1;
-      else hi = mid;
-    }
-    pos = lbeg + (g - lbeg) + (lo - rbeg);
-  } else {
-    // right element: count left elements less or equal
-    int64_t lo = lbeg, hi = lend;
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if (!kless(key, kin[mid])) lo = mid + 1;
-      else hi = mid;
-    }
-    pos = lbeg + (g - rbeg) + (lo - lbeg);
-  }
-  kout[pos] = key;
-  vout[pos] = vin[g];
+// workspace of a radix sort of R segments of stride S: two key buffers, one value buffer, the tile
+// counts and the digit totals
+struct RadixWs {
+  uint64_t* kA;
+  uint64_t* kB;
+  int32_t* vB;
+  uint32_t* counts;
+  uint32_t* totals;
+};
+
+static size_t radix_ws_bytes(int R, int64_t S) {
+  const int64_t T = (S + RS_TILE - 1) / RS_TILE;
+  size_t o = 0;
+  o += (size_t)R * S * 8 * 2;
+  o = (o + 255) & ~(size_t)255;
+  o += (size_t)R * S * 4;
+  o = (o + 255) & ~(size_t)255;
+  o += (size_t)R * RS_BINS * T * 4;
+  o = (o + 255) & ~(size_t)255;
+  o += (size_t)R * RS_BINS * 4 + 256;
+  return o;
 }
 
-size_t argsort_ws(int64_t n) { return (size_t)n * (2 * sizeof(double) + sizeof(int32_t)) + 512; }
+static RadixWs radix_ws(void* ws, int R, int64_t S) {
+  const int64_t T = (S + RS_TILE - 1) / RS_TILE;
+  char* p = (char*)ws;
+  RadixWs w;
+  w.kA = (uint64_t*)p;
+  w.kB = w.kA + (size_t)R * S;
+  size_t o = (size_t)R * S * 8 * 2;
+  o = (o + 255) & ~(size_t)255;
+  w.vB = (int32_t*)(p + o);
+  o += (size_t)R * S * 4;
+  o = (o + 255) & ~(size_t)255;
+  w.counts = (uint32_t*)(p + o);
+  o += (size_t)R * RS_BINS * T * 4;
+  o = (o + 255) & ~(size_t)255;
+  w.totals = (uint32_t*)(p + o);
+  return w;
+}
 
-// perm receives int32 indices; workspace holds two key buffers and one index buffer
-hipError_t argsort_stable(const double* keys, int64_t n, int32_t* perm, void* ws, size_t ws_bytes, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  double* kA = (double*)ws;
-  double* kB = kA + n;
-  int32_t* vB = (int32_t*)(kB + n);
-  int64_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
-  // ping-pong between (kA, perm) and (kB, vB); arrange the final pass to land in perm
-  int passes = 0;
-  for (int64_t r = SORT_TILE; r < n; r <<= 1) ++passes;
-  double* k0 = (passes % 2 == 0) ? kA : kB;
-  int32_t* v0 = (passes % 2 == 0) ? perm : vB;
-  double* k1 = (passes % 2 == 0) ? kB : kA;
-  int32_t* v1 = (passes % 2 == 0) ? vB : perm;
-  hipLaunchKernelGGL(k_tile_sort, dim3((unsigned)tiles), dim3(SORT_THREADS), 0, s, keys, n, k0, v0);
-  for (int64_t r = SORT_TILE; r < n; r <<= 1) {
-    hipLaunchKernelGGL(k_merge_pass, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, k0, v0, n, r, k1, v1);
-    double* tk = k0;
-    k0 = k1;
-    k1 = tk;
-    int32_t* tv = v0;
-    v0 = v1;
-    v1 = tv;
+// sorts every segment r (keys[r * S + i], i < seg.len(r)) and writes its permutation to perm[r * S + i];
+// keys are read, not written; perm also serves as the value buffer of the odd passes
+template <class Seg>
+static hipError_t radix_argsort(const double* keys, int R, int64_t S, Seg seg, int32_t* perm, const RadixWs& w,
+                                hipStream_t s) {
+  const int T = (int)((S + RS_TILE - 1) / RS_TILE);
+  const dim3 grid((unsigned)T, (unsigned)R);
+  // pass p writes keys to kB (p even) / kA (p odd) and values to vB (p even) / perm (p odd); the last
+  // pass (odd) reads kB / vB and writes values to perm
+  for (int p = 0; p < 8; ++p) {
+    const int shift = 8 * p;
+    const void* kin = p == 0 ? (const void*)keys : (const void*)((p & 1) ? w.kB : w.kA);
+    const int32_t* vin = (p & 1) ? w.vB : perm;
+    uint64_t* kout = (p & 1) ? w.kA : w.kB;
+    int32_t* vout = (p & 1) ? perm : w.vB;
+    if (p == 0)
+      hipLaunchKernelGGL((k_rs_count<true, Seg>), grid, dim3(RS_THREADS), 0, s, kin, S, T, seg, shift, w.counts);
+    else
+      hipLaunchKernelGGL((k_rs_count<false, Seg>), grid, dim3(RS_THREADS), 0, s, kin, S, T, seg, shift, w.counts);
+    hipLaunchKernelGGL(k_rs_scan, dim3(RS_BINS, (unsigned)R), dim3(RS_THREADS), 0, s, w.counts, T, w.totals);
+    if (p == 0)
+      hipLaunchKernelGGL((k_rs_scatter<true, false, Seg>), grid, dim3(RS_THREADS), 0, s, kin, vin, S, T, seg, shift,
+                         w.counts, w.totals, kout, vout);
+    else if (p < 7)
+      hipLaunchKernelGGL((k_rs_scatter<false, false, Seg>), grid, dim3(RS_THREADS), 0, s, kin, vin, S, T, seg, shift,
+                         w.counts, w.totals, kout, vout);
+    else
+      hipLaunchKernelGGL((k_rs_scatter<false, true, Seg>), grid, dim3(RS_THREADS), 0, s, kin, vin, S, T, seg, shift,
+                         w.counts, w.totals, kout, vout);
   }
   return hipGetLastError();
+}
+
+size_t argsort_ws(int64_t n) { return radix_ws_bytes(1, n < 1 ? 1 : n); }
+
+// perm receives int32 indices
+hipError_t argsort_stable(const double* keys, int64_t n, int32_t* perm, void* ws, size_t ws_bytes, hipStream_t s) {
+  (void)ws_bytes;
+  if (n == 0) return hipSuccess;
+  return radix_argsort(keys, 1, n, SegFixed{n}, perm, radix_ws(ws, 1, n), s);
 }
 
 // ===========================================================================
@@ -488,11 +675,6 @@ hipError_t count_crossed_edges(const int32_t* sender, const double* logL, int64_
 // max_samples) read on the device; per-run segments of the workspace have a
 // common stride S >= max_r(M_r) + 1.
 // ===========================================================================
-__device__ __forceinline__ int64_t batch_M(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int r) {
-  const int64_t ns = states[r].num_samples, ms = runs[r].max_samples;
-  return ns < ms ? ns : ms;
-}
-
 __global__ void k_b_nodes(const hipns_run_desc_t* runs, const hipns_run_state_t* states, const double* samp_logL,
                           int64_t S, double* nodes) {
   const int r = blockIdx.y;
@@ -500,86 +682,6 @@ __global__ void k_b_nodes(const hipns_run_desc_t* runs, const hipns_run_state_t*
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i > M) return;
   nodes[r * S + i] = i == 0 ? -__builtin_inf() : samp_logL[runs[r].samp_off + i - 1];
-}
-
-__global__ void __launch_bounds__(SORT_THREADS) k_b_tile_sort(const hipns_run_desc_t* runs,
-                                                              const hipns_run_state_t* states, const double* keys_in,
-                                                              int64_t S, double* keys_out, int32_t* idx_out) {
-  __shared__ double k[SORT_TILE];
-  __shared__ int32_t v[SORT_TILE];
-  const int r = blockIdx.y;
-  const int64_t n = batch_M(runs, states, r) + 1;
-  const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-  if (base >= n) return;  // block-uniform
-  const double* kin = keys_in + r * S;
-  for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
-    int64_t g = base + i;
-    k[i] = g < n ? kin[g] : __builtin_nan("");
-    v[i] = g < n ? (int32_t)g : (int32_t)0x7fffffff;
-  }
-  __syncthreads();
-  for (int kk = 2; kk <= SORT_TILE; kk <<= 1) {
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
-        int p = i ^ j;
-        if (p > i) {
-          bool up = (i & kk) == 0;
-          double ki = k[i], kp = k[p];
-          int32_t ii = v[i], ip = v[p];
-          bool sw = up ? key_lt(kp, ip, ki, ii) : key_lt(ki, ii, kp, ip);
-          if (sw) {
-            k[i] = kp;
-            k[p] = ki;
-            v[i] = ip;
-            v[p] = ii;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
-    int64_t g = base + i;
-    if (g < n) {
-      keys_out[r * S + g] = k[i];
-      idx_out[r * S + g] = v[i];
-    }
-  }
-}
-
-__global__ void k_b_merge_pass(const hipns_run_desc_t* runs, const hipns_run_state_t* states, const double* kin_all,
-                               const int32_t* vin_all, int64_t S, int64_t run, double* kout_all, int32_t* vout_all) {
-  const int r = blockIdx.y;
-  const int64_t n = batch_M(runs, states, r) + 1;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  const double* kin = kin_all + r * S;
-  int64_t pair = g / (2 * run);
-  int64_t lbeg = pair * 2 * run;
-  int64_t rbeg = lbeg + run;
-  int64_t lend = rbeg < n ? rbeg : n;
-  int64_t rend = rbeg + run < n ? rbeg + run : n;
-  double key = kin[g];
-  int64_t pos;
-  if (g < rbeg) {
-    int64_t lo = rbeg, hi = rend;
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if (kless(kin[mid], key)) lo = mid + 1;
-      else hi = mid;
-    }
-    pos = lbeg + (g - lbeg) + (lo - rbeg);
-  } else {
-    int64_t lo = lbeg, hi = lend;
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if (!kless(key, kin[mid])) lo = mid + 1;
-      else hi = mid;
-    }
-    pos = lbeg + (g - rbeg) + (lo - lbeg);
-  }
-  kout_all[r * S + pos] = key;
-  vout_all[r * S + pos] = vin_all[r * S + g];
 }
 
 // out-degree of every node by sender (tree_structure.py:33-108) and the phantom count of each run
@@ -713,12 +815,12 @@ static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 struct BatchEvLayout {
   int64_t S, Tscan, Tev;
   size_t zero_bytes, total;
-  size_t off_status, off_counters, off_kA, off_kB, off_vA, off_vB, off_nl, off_ts;
+  size_t off_status, off_counters, off_perm, off_radix, off_nl, off_ts;
 };
 
 static BatchEvLayout batch_ev_layout(int R, int64_t max_M) {
   BatchEvLayout L{};
-  L.S = round_up(max_M + 1, SORT_TILE);
+  L.S = round_up(max_M + 1, RS_TILE);
   L.Tscan = (L.S + SCAN_TILE - 1) / SCAN_TILE;
   L.Tev = (L.S + EV_TILE - 1) / EV_TILE;
   size_t o = 0;
@@ -731,14 +833,11 @@ static BatchEvLayout batch_ev_layout(int R, int64_t max_M) {
   o += (size_t)R * sizeof(uint32_t);
   o = round_up(o, 256);
   L.zero_bytes = o;
-  L.off_kA = o;
-  o += (size_t)R * L.S * sizeof(double);
-  L.off_kB = o;
-  o += (size_t)R * L.S * sizeof(double);
-  L.off_vA = o;
+  L.off_perm = o;
   o += (size_t)R * L.S * sizeof(int32_t);
-  L.off_vB = o;
-  o += (size_t)R * L.S * sizeof(int32_t);
+  o = round_up(o, 256);
+  L.off_radix = o;  // the node keys are written into the radix sort's first key buffer
+  o += radix_ws_bytes(R, L.S);
   o = round_up(o, 256);
   L.off_nl = o;
   o += (size_t)R * L.S * sizeof(int64_t);
@@ -757,32 +856,20 @@ hipError_t batch_evidence(const hipns_run_desc_t* runs, const hipns_run_state_t*
   int32_t* deg = (int32_t*)base;
   TileStatus* status = (TileStatus*)(base + L.off_status);
   uint32_t* counters = (uint32_t*)(base + L.off_counters);
-  double* kA = (double*)(base + L.off_kA);
-  double* kB = (double*)(base + L.off_kB);
-  int32_t* vA = (int32_t*)(base + L.off_vA);
-  int32_t* vB = (int32_t*)(base + L.off_vB);
+  int32_t* perm = (int32_t*)(base + L.off_perm);
+  const RadixWs rw = radix_ws(base + L.off_radix, R, L.S);
+  double* nodes = (double*)rw.kA;
   int64_t* nl = (int64_t*)(base + L.off_nl);
   int64_t* ts = (int64_t*)(base + L.off_ts);
   const int64_t words = (int64_t)(L.zero_bytes / 4);
   hipLaunchKernelGGL(k_fill_zero_u32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, (uint32_t*)base, words);
   hipLaunchKernelGGL(k_fill_zero_u32, dim3((unsigned)((2 * R + 255) / 256)), dim3(256), 0, s, (uint32_t*)phantom,
                      (int64_t)2 * R);
-  // nodes into kA; sort: tile pass writes (k0, v0), merges ping-pong; the final buffers hold the permutation
+  // node keys (root -inf, then the samples), sorted per run by the radix sort into perm
   hipLaunchKernelGGL(k_b_nodes, dim3((unsigned)((L.S + 255) / 256), R), dim3(256), 0, s, runs, states, B.samp_logL,
-                     L.S, kA);
-  const int64_t tiles = L.S / SORT_TILE;
-  hipLaunchKernelGGL(k_b_tile_sort, dim3((unsigned)tiles, R), dim3(SORT_THREADS), 0, s, runs, states, kA, L.S, kB, vB);
-  double* k0 = kB;
-  int32_t* v0 = vB;
-  double* k1 = kA;
-  int32_t* v1 = vA;
-  for (int64_t run = SORT_TILE; run < L.S; run <<= 1) {
-    hipLaunchKernelGGL(k_b_merge_pass, dim3((unsigned)((L.S + 255) / 256), R), dim3(256), 0, s, runs, states, k0, v0,
-                       L.S, run, k1, v1);
-    std::swap(k0, k1);
-    std::swap(v0, v1);
-  }
-  const int32_t* perm = v0;
+                     L.S, nodes);
+  hipError_t e = radix_argsort(nodes, R, L.S, SegBatchNodes{runs, states}, perm, rw, s);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_b_out_degree, dim3((unsigned)((L.S + 255) / 256), R), dim3(256), 0, s, runs, states, B, L.S,
                      deg, (unsigned long long*)phantom);
   hipLaunchKernelGGL(k_b_scan_tiles, dim3((unsigned)L.Tscan, R), dim3(SCAN_THREADS), 0, s, runs, states, deg, perm,

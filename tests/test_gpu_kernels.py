@@ -141,6 +141,39 @@ def test_argsort_stable_vs_numpy():
         assert np.array_equal(p.cpu().numpy(), np.argsort(keys, kind="stable"))
 
 
+def test_argsort_stable_edge_keys():
+    """Keys where an order of bit patterns and the comparison order part: -0 and +0 tie (input order
+    kept), NaNs of either sign and any payload tie after +inf, subnormals and both signs of every
+    magnitude, all-equal keys, and sizes spanning many 2048-element tiles with ragged ends."""
+    torch = _torch()
+    rng = np.random.default_rng(17)
+
+    def run(keys):
+        n = keys.size
+        k = torch.from_numpy(keys).cuda()
+        p = torch.empty(n, dtype=torch.int64, device="cuda")
+        ws = torch.empty(lib().hipns_argsort_workspace_size(n), dtype=torch.uint8, device="cuda")
+        check(lib().hipns_argsort_stable(ptr(k), n, ptr(p), ptr(ws), ws.numel(), _stream()))
+        got = p.cpu().numpy()
+        assert np.array_equal(got, np.argsort(keys, kind="stable")), f"n={n}"
+
+    specials = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 5e-324, -5e-324, 2.2e-308, -2.2e-308,
+                         1.7976931348623157e308, -1.7976931348623157e308, 1.0, -1.0])
+    nan_payload = np.array([0x7FF0000000000001, 0xFFF8000000000123, 0x7FFFFFFFFFFFFFFF], dtype=np.uint64).view(np.float64)
+    for n in [3, 2047, 4097, 65536 + 11, (1 << 21) + 3]:
+        say(f"argsort edge keys n={n}")
+        keys = rng.choice(np.concatenate([specials, nan_payload]), size=n)
+        mask = rng.random(n) < 0.5
+        keys[mask] = rng.normal(size=int(mask.sum())) * 10.0 ** rng.integers(-300, 300, size=int(mask.sum()))
+        run(keys)
+    say("argsort all-equal keys")
+    run(np.full(10000, -0.0))
+    run(np.zeros(4096 * 3 + 1))
+    say("argsort large ragged")
+    keys = np.round(rng.normal(size=(1 << 22) + 1234), 2)
+    run(keys)
+
+
 def _tree(n, seed):
     rng = np.random.default_rng(seed)
     logL = [0.0]
