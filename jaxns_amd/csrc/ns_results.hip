@@ -238,7 +238,7 @@ hipError_t evidence_scan(const double* logL, const double* n, int64_t M, int64_t
 // length; nothing waits on another workgroup.
 // ===========================================================================
 constexpr int RS_THREADS = 256;  // == RS_BINS: thread d owns digit d in the per-digit steps
-constexpr int RS_ITEMS = 8;
+constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_BINS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
@@ -321,11 +321,17 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_count(const void* kin, int64_
   const int64_t base = (int64_t)t * RS_TILE;
   if (base < n) {  // block-uniform
     const uint64_t lt = (1ull << lane_id()) - 1ull;
+    // all loads first (clamped in-bounds index, so they issue together), then the histogram
+    uint64_t kr[RS_ITEMS];
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; ++k) {
       const int64_t i = base + k * RS_THREADS + tid;
-      const bool valid = i < n;
-      const uint32_t d = valid ? (uint32_t)(rs_key<FIRST>(kin, (int64_t)r * S + i) >> shift) & 255u : 0u;
+      kr[k] = rs_key<FIRST>(kin, (int64_t)r * S + (i < n ? i : n - 1));
+    }
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) {
+      const bool valid = base + k * RS_THREADS + tid < n;
+      const uint32_t d = (uint32_t)(kr[k] >> shift) & 255u;
       const uint64_t peers = digit_peers(d, valid);
       if (valid && (peers & lt) == 0ull) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     }
@@ -389,14 +395,21 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const void* kin, cons
   uint64_t kr[RS_ITEMS];
   int32_t vr[RS_ITEMS];
   uint32_t dr[RS_ITEMS];  // digit | (rank among the tile's elements of that digit) << 8; ~0 = none
+  // all loads first (clamped in-bounds index, so they issue together), then the ranking rounds
+#pragma unroll
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    const int64_t i = base + k * RS_THREADS + tid;
+    const int64_t ic = i < n ? i : n - 1;
+    kr[k] = rs_key<FIRST>(kin, so + ic);
+    vr[k] = FIRST ? (int32_t)i : vin[so + ic];
+  }
 #pragma unroll
   for (int k = 0; k < RS_ITEMS; ++k) {
     const int buf = k & 1;
     const int64_t i = base + k * RS_THREADS + tid;
     const bool valid = i < n;
-    const uint64_t key = valid ? rs_key<FIRST>(kin, so + i) : ~0ull;
-    int32_t val = 0;
-    if (valid) val = FIRST ? (int32_t)i : vin[so + i];
+    const uint64_t key = kr[k];
+    const int32_t val = vr[k];
     const uint32_t d = (uint32_t)(key >> shift) & 255u;
     const uint64_t peers = digit_peers(d, valid);
     const uint32_t rw = (uint32_t)__popcll(peers & lt);
@@ -551,14 +564,21 @@ __device__ __forceinline__ int64_t crossed_term(const int32_t* deg, const int32_
   return (int64_t)deg[perm[i]] - 1;
 }
 
+// gathers the sorted crossed-edge terms deg[perm[i]] - 1 once (zero from `stop` on) into term[] and sums
+// each tile (strided, coalesced: a sum does not need the element order)
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int32_t* deg, const int32_t* perm, int64_t n,
-                                                             int64_t stop, int64_t* tile_sums) {
+                                                             int64_t stop, int64_t* tile_sums, int32_t* term) {
   __shared__ int64_t red[SCAN_THREADS / 64];
   int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
   int64_t acc = 0;
+#pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = base + (int64_t)threadIdx.x * SCAN_ITEMS + k;
-    if (i < n && i < stop) acc += crossed_term(deg, perm, i);
+    int64_t i = base + (int64_t)k * SCAN_THREADS + threadIdx.x;
+    if (i < n) {
+      const int32_t v = i < stop ? deg[perm[i]] - 1 : 0;
+      term[i] = v;
+      acc += v;
+    }
   }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
   if (lane_id() == 0) red[threadIdx.x / 64] = acc;
@@ -595,7 +615,7 @@ __global__ void k_scan_sums(int64_t* tile_sums, int64_t T) {
   }
 }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const int32_t* deg, const int32_t* perm, int64_t n,
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const int32_t* term, const int32_t* perm, int64_t n,
                                                              int64_t stop, int64_t fake, const int64_t* tile_sums,
                                                              int64_t* num_live, int64_t* samples_indices) {
   __shared__ int64_t tsum[SCAN_THREADS];
@@ -604,7 +624,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const int32_t* deg,
   int64_t acc = 0;
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     int64_t i = base + k;
-    int64_t v = (i < n && i < stop) ? crossed_term(deg, perm, i) : 0;
+    int64_t v = i < n ? term[i] : 0;
     acc += v;
     local[k] = acc;
   }
@@ -633,7 +653,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const int32_t* deg,
 size_t count_crossed_edges_ws(int64_t N) {
   int64_t n = N + 1;
   int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  return (size_t)n * sizeof(double) + (size_t)n * sizeof(int32_t) * 2 + (size_t)tiles * sizeof(int64_t) +
+  return (size_t)n * sizeof(double) + (size_t)n * sizeof(int32_t) * 3 + (size_t)tiles * sizeof(int64_t) +
          argsort_ws(n) + 1024;
 }
 
@@ -646,6 +666,8 @@ hipError_t count_crossed_edges(const int32_t* sender, const double* logL, int64_
   int32_t* perm = (int32_t*)p;
   p += n * sizeof(int32_t);
   int32_t* deg = (int32_t*)p;
+  p += n * sizeof(int32_t);
+  int32_t* term = (int32_t*)p;
   p += n * sizeof(int32_t);
   p = (char*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
   int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -661,9 +683,9 @@ hipError_t count_crossed_edges(const int32_t* sender, const double* logL, int64_
     hipLaunchKernelGGL(k_out_degree, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, sender, N, deg);
   const int64_t stop = num_samples < 0 ? n : num_samples;
   const int64_t fake = num_samples < 0 ? 0 : (N - num_samples);
-  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, deg, perm, n, stop, tsums);
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, deg, perm, n, stop, tsums, term);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tsums, tiles);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, deg, perm, n, stop, fake, tsums,
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, term, perm, n, stop, fake, tsums,
                      num_live, samples_indices);
   return hipGetLastError();
 }
