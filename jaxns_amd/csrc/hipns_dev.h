@@ -391,6 +391,10 @@ __device__ __forceinline__ EvMap shfl_xor_map(const EvMap& m, int o) {
   return {__shfl_xor(m.a, o, 64), __shfl_xor(m.b, o, 64), __shfl_xor(m.g, o, 64), __shfl_xor(m.e, o, 64),
           __shfl_xor(m.z, o, 64), __shfl_xor(m.h, o, 64), __shfl_xor(m.th, o, 64)};
 }
+__device__ __forceinline__ EvMap shfl_down_map(const EvMap& m, int o) {
+  return {__shfl_down(m.a, o, 64), __shfl_down(m.b, o, 64), __shfl_down(m.g, o, 64), __shfl_down(m.e, o, 64),
+          __shfl_down(m.z, o, 64), __shfl_down(m.h, o, 64), __shfl_down(m.th, o, 64)};
+}
 __device__ __forceinline__ EvMap shfl_up_map(const EvMap& m, int o) {
   return {__shfl_up(m.a, o, 64), __shfl_up(m.b, o, 64), __shfl_up(m.g, o, 64), __shfl_up(m.e, o, 64),
           __shfl_up(m.z, o, 64), __shfl_up(m.h, o, 64), __shfl_up(m.th, o, 64)};

@@ -68,12 +68,13 @@ struct EvSrcSorted {
   __device__ double N(int64_t i) const { return (double)nl[i]; }
 };
 
-template <class Src>
+template <bool PER8, class Src>
 __device__ __forceinline__ void evidence_scan_tile(const Src& src, int64_t M, int64_t stop, double* per8, double* fin8,
                                                    TileStatus* status, uint32_t* tile_counter) {
   __shared__ EvMap wave_aggs[EV_THREADS / 64];
   __shared__ EvMap s_excl;
   __shared__ uint32_t s_tile;
+  __shared__ double s_stage[EV_TILE];
   if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
   __syncthreads();
   const int64_t tile = s_tile;
@@ -119,36 +120,57 @@ __device__ __forceinline__ void evidence_scan_tile(const Src& src, int64_t M, in
   if (l == 0) lane_excl = evmap_identity();
   EvMap thr_excl = evmap_then(wave_prefix, lane_excl);
 
-  // tile aggregate and decoupled look-back (one thread)
-  if (tid == 0) {
+  // tile aggregate and decoupled look-back by the first wavefront.  Lane j inspects predecessor tile
+  // top - j (64 tiles at a time, each lane waiting only for that tile's aggregate, which every claimed
+  // tile publishes before it looks back itself).  The nearest tile with an inclusive prefix ends the
+  // walk; the maps from it up to the tile before this one are composed by an ordered tree reduction
+  // across the lanes (a higher lane is an earlier tile, so it is applied first).
+  if (w == 0) {
     EvMap tile_agg = evmap_identity();
     for (int i = 0; i < EV_THREADS / 64; ++i) tile_agg = evmap_then(tile_agg, wave_aggs[i]);
     EvMap excl = evmap_identity();
     if (tile == 0) {
-      store_map(&status[0].incl, tile_agg);
-      __hip_atomic_store(&status[0].flag, 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (l == 0) {
+        store_map(&status[0].incl, tile_agg);
+        __hip_atomic_store(&status[0].flag, 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
     } else {
-      store_map(&status[tile].agg, tile_agg);
-      __hip_atomic_store(&status[tile].flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      EvMap run = evmap_identity();
-      int64_t t = tile - 1;
+      if (l == 0) {
+        store_map(&status[tile].agg, tile_agg);
+        __hip_atomic_store(&status[tile].flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      EvMap run = evmap_identity();  // composition of the tiles already folded in (the later ones)
+      int64_t top = tile - 1;
       while (true) {
-        int f;
-        do {
-          f = __hip_atomic_load(&status[t].flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        } while (f == 0);
-        if (f == 2) {
-          run = evmap_then(load_map(&status[t].incl), run);
-          break;
+        const int64_t tt = top - l;
+        int f = 2;  // before tile 0: never reached, tile 0 publishes its inclusive prefix directly
+        if (tt >= 0) {
+          do {
+            f = __hip_atomic_load(&status[tt].flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          } while (f == 0);
         }
-        run = evmap_then(load_map(&status[t].agg), run);
-        --t;
+        const uint64_t incl_lanes = __ballot(f == 2);
+        const int jstar = incl_lanes ? __ffsll((unsigned long long)incl_lanes) - 1 : 64;
+        EvMap mine = evmap_identity();
+        if (l < jstar) mine = load_map(&status[tt].agg);
+        else if (l == jstar) mine = load_map(&status[tt].incl);
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const EvMap later = mine;
+          const EvMap earlier = shfl_down_map(mine, o);
+          if (l + o < 64) mine = evmap_then(earlier, later);
+        }
+        run = evmap_then(mine, run);  // lane 0's value is the window's composition
+        if (jstar < 64) break;
+        top -= 64;
       }
       excl = run;
-      store_map(&status[tile].incl, evmap_then(excl, tile_agg));
-      __hip_atomic_store(&status[tile].flag, 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (l == 0) {
+        store_map(&status[tile].incl, evmap_then(excl, tile_agg));
+        __hip_atomic_store(&status[tile].flag, 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
-    s_excl = excl;
+    if (l == 0) s_excl = excl;
   }
   __syncthreads();
   const EvMap pre = evmap_then(s_excl, thr_excl);
@@ -158,47 +180,59 @@ __device__ __forceinline__ void evidence_scan_tile(const Src& src, int64_t M, in
   s = evmap_apply(pre, s_init);
   s.L = (base == 0) ? ninf : (base - 1 < stop ? src.L(base - 1) : ninf);
   s.dZ = ninf;
+  EvState out[PER8 ? EV_ITEMS : 1];
 #pragma unroll
   for (int k = 0; k < EV_ITEMS; ++k) {
     int64_t i = base + k;
-    if (i < M) {
-      EvState o;
-      if (i < stop) {
-        o = evmap_apply(item[k], s);
-        o.L = item_L[k];
-        o.dZ = item_lt[k] + s.X;
-        s = o;
-      } else {
-        o = s_init;
+    EvState o = s_init;
+    if (i < M && i < stop) {
+      o = evmap_apply(item[k], s);
+      o.L = item_L[k];
+      o.dZ = item_lt[k] + s.X;
+      s = o;
+    }
+    if constexpr (PER8) out[k] = o;
+    if (fin8 && i < M && i == (stop < M ? stop : M) - 1) {
+      fin8[0] = o.L;
+      fin8[1] = o.X;
+      fin8[2] = o.X2;
+      fin8[3] = o.Z;
+      fin8[4] = o.ZX;
+      fin8[5] = o.Z2;
+      fin8[6] = o.dZ;
+      fin8[7] = o.dZ2;
+    }
+  }
+  if constexpr (PER8) {
+    // field by field through a per-wavefront LDS stage, so that each store writes 64 consecutive
+    // samples (the thread's own samples are consecutive, 8 doubles apart between lanes)
+    double* stage = s_stage + w * 64 * EV_ITEMS;
+    const int64_t wbase = tile * EV_TILE + (int64_t)w * 64 * EV_ITEMS;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+#pragma unroll
+      for (int k = 0; k < EV_ITEMS; ++k) {
+        const EvState& o = out[k];
+        const double v = f == 0 ? o.L : f == 1 ? o.X : f == 2 ? o.X2 : f == 3 ? o.Z : f == 4 ? o.ZX
+                       : f == 5 ? o.Z2 : f == 6 ? o.dZ : o.dZ2;
+        stage[l * EV_ITEMS + k] = v;
       }
-      if (per8) {
-        per8[0 * M + i] = o.L;
-        per8[1 * M + i] = o.X;
-        per8[2 * M + i] = o.X2;
-        per8[3 * M + i] = o.Z;
-        per8[4 * M + i] = o.ZX;
-        per8[5 * M + i] = o.Z2;
-        per8[6 * M + i] = o.dZ;
-        per8[7 * M + i] = o.dZ2;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < EV_ITEMS; ++k) {
+        const int64_t i = wbase + k * 64 + l;
+        if (i < M) per8[(int64_t)f * M + i] = stage[k * 64 + l];
       }
-      if (fin8 && i == (stop < M ? stop : M) - 1) {
-        fin8[0] = o.L;
-        fin8[1] = o.X;
-        fin8[2] = o.X2;
-        fin8[3] = o.Z;
-        fin8[4] = o.ZX;
-        fin8[5] = o.Z2;
-        fin8[6] = o.dZ;
-        fin8[7] = o.dZ2;
-      }
+      __syncthreads();
     }
   }
 }
 
+template <bool PER8>
 __global__ void __launch_bounds__(EV_THREADS) k_evidence_scan(const double* logL, const double* nlive, int64_t M,
                                                               int64_t stop, double* per8, double* fin8,
                                                               TileStatus* status, uint32_t* tile_counter) {
-  evidence_scan_tile(EvSrcFlat{logL, nlive}, M, stop, per8, fin8, status, tile_counter);
+  evidence_scan_tile<PER8>(EvSrcFlat{logL, nlive}, M, stop, per8, fin8, status, tile_counter);
 }
 
 __global__ void k_fin_init(double* fin8) {
@@ -223,8 +257,12 @@ hipError_t evidence_scan(const double* logL, const double* n, int64_t M, int64_t
   TileStatus* status = (TileStatus*)((char*)ws + 256);
   int64_t words = (256 + tiles * (int64_t)sizeof(TileStatus)) / 4;
   hipLaunchKernelGGL(k_fill_zero_u32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, (uint32_t*)ws, words);
-  hipLaunchKernelGGL(k_evidence_scan, dim3((unsigned)tiles), dim3(EV_THREADS), 0, s, logL, n, M, stop, per8,
-                     stop > 0 ? fin8 : nullptr, status, counter);
+  if (per8)
+    hipLaunchKernelGGL(k_evidence_scan<true>, dim3((unsigned)tiles), dim3(EV_THREADS), 0, s, logL, n, M, stop, per8,
+                       stop > 0 ? fin8 : nullptr, status, counter);
+  else
+    hipLaunchKernelGGL(k_evidence_scan<false>, dim3((unsigned)tiles), dim3(EV_THREADS), 0, s, logL, n, M, stop,
+                       per8, stop > 0 ? fin8 : nullptr, status, counter);
   return hipGetLastError();
 }
 
@@ -619,7 +657,15 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const int32_t* term
                                                              int64_t stop, int64_t fake, const int64_t* tile_sums,
                                                              int64_t* num_live, int64_t* samples_indices) {
   __shared__ int64_t tsum[SCAN_THREADS];
-  int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  __shared__ int64_t s_out[SCAN_TILE];
+  const int64_t tbase = (int64_t)blockIdx.x * SCAN_TILE;
+  // samples_indices[i] = perm[i + 1] - 1 (tree_structure.py:104), written directly (coalesced)
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t i = tbase + (int64_t)k * SCAN_THREADS + threadIdx.x;
+    if (i < n - 1) samples_indices[i] = (int64_t)perm[i + 1] - 1;
+  }
+  int64_t base = tbase + (int64_t)threadIdx.x * SCAN_ITEMS;
   int64_t local[SCAN_ITEMS];
   int64_t acc = 0;
   for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -637,16 +683,18 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const int32_t* term
     __syncthreads();
   }
   int64_t pre = tile_sums[blockIdx.x] + tsum[threadIdx.x] - acc;
+  // crossed_edges_sorted[i] through an LDS stage, so that each store writes consecutive entries
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     int64_t i = base + k;
-    if (i >= n) break;
-    // crossed_edges_sorted[i]; the last node (i = n-1 = N) is dropped
     int64_t crossed = (i < stop) ? (1 + pre + local[k]) : fake;
-    crossed -= fake;
-    if (i < n - 1) {
-      num_live[i] = crossed;
-      samples_indices[i] = (int64_t)perm[i + 1] - 1;
-    }
+    s_out[threadIdx.x * SCAN_ITEMS + k] = crossed - fake;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int j = k * SCAN_THREADS + threadIdx.x;
+    const int64_t i = tbase + j;
+    if (i < n - 1) num_live[i] = s_out[j];  // the last node (i = n - 1 = N) is dropped
   }
 }
 
@@ -829,7 +877,7 @@ __global__ void __launch_bounds__(EV_THREADS) k_b_evidence_scan(const hipns_run_
     return;
   }
   const EvSrcSorted src{samp_logL + runs[r].samp_off, perm + r * S, num_live + r * S};
-  evidence_scan_tile(src, M, M, nullptr, fin8, status + r * Tev, counters + r);
+  evidence_scan_tile<false>(src, M, M, nullptr, fin8, status + r * Tev, counters + r);
 }
 
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
