@@ -9,47 +9,32 @@ namespace hipns {
 
 // ===========================================================================
 // Evidence scan: compute_evidence_stats (internals/shrinkage_statistics.py:131-157)
-// as a single-pass decoupled look-back scan over 7-coefficient log-space maps.
-// Each 256-thread block owns a tile of 256*ITEMS consecutive samples; tiles
-// are claimed in order from an atomic counter so that look-back only waits on
-// tiles that are already running.
+// as a reduce-then-scan over 7-coefficient log-space maps, in three launches
+// (grid.y = segment, so one set of launches serves every run of a batch):
+//   1. k_ev_reduce: each 256-thread block reduces a tile of 256*ITEMS
+//      consecutive samples (each thread composes its ITEMS step maps, then an
+//      ordered scan across the block) and stores the tile's aggregate and
+//      every thread's exclusive prefix within the tile;
+//   2. k_ev_scan_tiles: one block per segment scans the tile aggregates
+//      (exclusive, in place);
+//   3. k_ev_apply: each thread applies its prefix to the initial state and
+//      steps through its samples, forming each step map again.
+// The grouping of the compositions depends only on the sample count, so a
+// segment's results do not depend on the other segments or on timing.
 // ===========================================================================
 constexpr int EV_THREADS = 256;
 constexpr int EV_ITEMS = 8;
 constexpr int EV_TILE = EV_THREADS * EV_ITEMS;
-
-struct TileStatus {
-  EvMap agg;
-  EvMap incl;
-  int flag;  // 0 = empty, 1 = aggregate published, 2 = inclusive prefix published
-  int pad;
-};
-
-__device__ __forceinline__ double ld_relaxed(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_relaxed(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ EvMap load_map(const EvMap* m) {
-  const double* p = (const double*)m;
-  return {ld_relaxed(p + 0), ld_relaxed(p + 1), ld_relaxed(p + 2), ld_relaxed(p + 3),
-          ld_relaxed(p + 4), ld_relaxed(p + 5), ld_relaxed(p + 6)};
-}
-__device__ __forceinline__ void store_map(EvMap* m, const EvMap& v) {
-  double* p = (double*)m;
-  st_relaxed(p + 0, v.a);
-  st_relaxed(p + 1, v.b);
-  st_relaxed(p + 2, v.g);
-  st_relaxed(p + 3, v.e);
-  st_relaxed(p + 4, v.z);
-  st_relaxed(p + 5, v.h);
-  st_relaxed(p + 6, v.th);
-}
+constexpr int EV_SCAN_THREADS = 1024;
 
 __global__ void k_fill_zero_u32(uint32_t* p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
+}
+
+__device__ __forceinline__ int64_t batch_M(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int r) {
+  const int64_t ns = states[r].num_samples, ms = runs[r].max_samples;
+  return ns < ms ? ns : ms;
 }
 
 // sources of the scan: (logL, n) at sorted position i
@@ -68,140 +53,164 @@ struct EvSrcSorted {
   __device__ double N(int64_t i) const { return (double)nl[i]; }
 };
 
-template <bool PER8, class Src>
-__device__ __forceinline__ void evidence_scan_tile(const Src& src, int64_t M, int64_t stop, double* per8, double* fin8,
-                                                   TileStatus* status, uint32_t* tile_counter) {
-  __shared__ EvMap wave_aggs[EV_THREADS / 64];
-  __shared__ EvMap s_excl;
-  __shared__ uint32_t s_tile;
-  __shared__ double s_stage[EV_TILE];
-  if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
-  __syncthreads();
-  const int64_t tile = s_tile;
-  if (tile * EV_TILE >= M) return;  // block-uniform: no tile left for this block
-  const int tid = threadIdx.x;
-  const int l = lane_id(), w = tid / 64;
-  const int64_t base = tile * EV_TILE + (int64_t)tid * EV_ITEMS;
-  const double ninf = -__builtin_inf();
-
-  // thread aggregate of its ITEMS consecutive elements
-  EvMap m = evmap_identity();
-  EvMap item[EV_ITEMS];
-  double item_lt[EV_ITEMS];
-  double item_L[EV_ITEMS];
-#pragma unroll
-  for (int k = 0; k < EV_ITEMS; ++k) {
-    int64_t i = base + k;
-    EvMap f = evmap_identity();
-    double lt = ninf, Li = ninf;
-    if (i < M && i < stop) {
-      double Lp = i == 0 ? ninf : src.L(i - 1);
-      Li = src.L(i);
-      f = evmap_step(src.N(i), Lp, Li, &lt);
-    }
-    item[k] = f;
-    item_lt[k] = lt;
-    item_L[k] = Li;
-    m = evmap_then(m, f);
+// segments: one flat array (samples past `stop` keep the initial state), or the sorted samples of
+// every run of a batch
+struct EvSegFlat {
+  const double* logL;
+  const double* n;
+  int64_t M, stop;
+  __device__ int64_t len(int) const { return M; }
+  __device__ int64_t stop_at(int) const { return stop; }
+  __device__ EvSrcFlat src(int) const { return {logL, n}; }
+};
+struct EvSegBatch {
+  const hipns_run_desc_t* runs;
+  const hipns_run_state_t* states;
+  const double* samp_logL;
+  const int32_t* perm;
+  const int64_t* num_live;
+  int64_t S;
+  __device__ int64_t len(int r) const { return batch_M(runs, states, r); }
+  __device__ int64_t stop_at(int r) const { return batch_M(runs, states, r); }
+  __device__ EvSrcSorted src(int r) const {
+    return {samp_logL + runs[r].samp_off, perm + (int64_t)r * S, num_live + (int64_t)r * S};
   }
-  // inclusive scan across the wavefront (ordered)
-  EvMap incl = m;
+};
+
+// ordered inclusive scan of one map per lane across the wavefront
+__device__ __forceinline__ EvMap wave_incl_scan(EvMap x) {
+  const int l = lane_id();
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    EvMap up = shfl_up_map(incl, o);
-    if (l >= o) incl = evmap_then(up, incl);
+    const EvMap up = shfl_up_map(x, o);
+    if (l >= o) x = evmap_then(up, x);
   }
+  return x;
+}
+
+template <class Seg>
+__global__ void __launch_bounds__(EV_THREADS) k_ev_reduce(Seg seg, int64_t T, EvMap* tile_agg, EvMap* thr_excl) {
+  __shared__ EvMap wave_aggs[EV_THREADS / 64];
+  const int r = blockIdx.y;
+  const int64_t M = seg.len(r), stop = seg.stop_at(r);
+  const int64_t tile = blockIdx.x;
+  if (tile * EV_TILE >= M) return;  // block-uniform
+  const auto src = seg.src(r);
+  const int tid = threadIdx.x, l = lane_id(), w = tid / 64;
+  const int64_t base = tile * EV_TILE + (int64_t)tid * EV_ITEMS;
+  const double ninf = -__builtin_inf();
+  EvMap m = evmap_identity();
+  for (int k = 0; k < EV_ITEMS; ++k) {
+    const int64_t i = base + k;
+    if (i < M && i < stop) {
+      double lt;
+      const double Lp = i == 0 ? ninf : src.L(i - 1);
+      m = evmap_then(m, evmap_step(src.N(i), Lp, src.L(i), &lt));
+    }
+  }
+  const EvMap incl = wave_incl_scan(m);
   if (l == 63) wave_aggs[w] = incl;
   __syncthreads();
-  // exclusive prefix of this thread within the tile
   EvMap wave_prefix = evmap_identity();
   for (int i = 0; i < w; ++i) wave_prefix = evmap_then(wave_prefix, wave_aggs[i]);
   EvMap lane_excl = shfl_up_map(incl, 1);
   if (l == 0) lane_excl = evmap_identity();
-  EvMap thr_excl = evmap_then(wave_prefix, lane_excl);
+  thr_excl[((int64_t)r * T + tile) * EV_THREADS + tid] = evmap_then(wave_prefix, lane_excl);
+  if (tid == 0) {
+    EvMap agg = evmap_identity();
+    for (int i = 0; i < EV_THREADS / 64; ++i) agg = evmap_then(agg, wave_aggs[i]);
+    tile_agg[(int64_t)r * T + tile] = agg;
+  }
+}
 
-  // tile aggregate and decoupled look-back by the first wavefront.  Lane j inspects predecessor tile
-  // top - j (64 tiles at a time, each lane waiting only for that tile's aggregate, which every claimed
-  // tile publishes before it looks back itself).  The nearest tile with an inclusive prefix ends the
-  // walk; the maps from it up to the tile before this one are composed by an ordered tree reduction
-  // across the lanes (a higher lane is an earlier tile, so it is applied first).
+// exclusive ordered scan of segment r's tile aggregates, in place: each thread composes a contiguous
+// chunk, the chunk totals are scanned across the block, then each thread walks its chunk again
+template <class Seg>
+__global__ void __launch_bounds__(EV_SCAN_THREADS) k_ev_scan_tiles(Seg seg, int64_t T, EvMap* tile_agg) {
+  constexpr int NW = EV_SCAN_THREADS / 64;
+  __shared__ EvMap wave_tot[NW];
+  const int r = blockIdx.y;
+  const int64_t Tr = (seg.len(r) + EV_TILE - 1) / EV_TILE;
+  if (Tr == 0) return;
+  EvMap* a = tile_agg + (int64_t)r * T;
+  const int tid = threadIdx.x, l = lane_id(), w = tid / 64;
+  const int64_t per = (Tr + EV_SCAN_THREADS - 1) / EV_SCAN_THREADS;
+  const int64_t j0 = tid * per;
+  const int64_t j1 = j0 + per < Tr ? j0 + per : Tr;
+  EvMap m = evmap_identity();
+  for (int64_t j = j0; j < j1; ++j) m = evmap_then(m, a[j]);
+  const EvMap incl = wave_incl_scan(m);
+  if (l == 63) wave_tot[w] = incl;
+  __syncthreads();
   if (w == 0) {
-    EvMap tile_agg = evmap_identity();
-    for (int i = 0; i < EV_THREADS / 64; ++i) tile_agg = evmap_then(tile_agg, wave_aggs[i]);
-    EvMap excl = evmap_identity();
-    if (tile == 0) {
-      if (l == 0) {
-        store_map(&status[0].incl, tile_agg);
-        __hip_atomic_store(&status[0].flag, 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    } else {
-      if (l == 0) {
-        store_map(&status[tile].agg, tile_agg);
-        __hip_atomic_store(&status[tile].flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      EvMap run = evmap_identity();  // composition of the tiles already folded in (the later ones)
-      int64_t top = tile - 1;
-      while (true) {
-        const int64_t tt = top - l;
-        int f = 2;  // before tile 0: never reached, tile 0 publishes its inclusive prefix directly
-        if (tt >= 0) {
-          do {
-            f = __hip_atomic_load(&status[tt].flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-          } while (f == 0);
-        }
-        const uint64_t incl_lanes = __ballot(f == 2);
-        const int jstar = incl_lanes ? __ffsll((unsigned long long)incl_lanes) - 1 : 64;
-        EvMap mine = evmap_identity();
-        if (l < jstar) mine = load_map(&status[tt].agg);
-        else if (l == jstar) mine = load_map(&status[tt].incl);
+    EvMap x = l < NW ? wave_tot[l] : evmap_identity();
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const EvMap later = mine;
-          const EvMap earlier = shfl_down_map(mine, o);
-          if (l + o < 64) mine = evmap_then(earlier, later);
-        }
-        run = evmap_then(mine, run);  // lane 0's value is the window's composition
-        if (jstar < 64) break;
-        top -= 64;
-      }
-      excl = run;
-      if (l == 0) {
-        store_map(&status[tile].incl, evmap_then(excl, tile_agg));
-        __hip_atomic_store(&status[tile].flag, 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
+    for (int o = 1; o < NW; o <<= 1) {
+      const EvMap up = shfl_up_map(x, o);
+      if (l >= o) x = evmap_then(up, x);
     }
-    if (l == 0) s_excl = excl;
+    EvMap ex = shfl_up_map(x, 1);
+    if (l == 0) ex = evmap_identity();
+    __syncwarp();
+    if (l < NW) wave_tot[l] = ex;
   }
   __syncthreads();
-  const EvMap pre = evmap_then(s_excl, thr_excl);
-  // state before this thread's first element
-  EvState s;
+  EvMap lane_excl = shfl_up_map(incl, 1);
+  if (l == 0) lane_excl = evmap_identity();
+  EvMap run = evmap_then(wave_tot[w], lane_excl);
+  for (int64_t j = j0; j < j1; ++j) {
+    const EvMap v = a[j];
+    a[j] = run;
+    run = evmap_then(run, v);
+  }
+}
+
+__device__ __forceinline__ void write_fin8(double* fin8, const EvState& o) {
+  fin8[0] = o.L;
+  fin8[1] = o.X;
+  fin8[2] = o.X2;
+  fin8[3] = o.Z;
+  fin8[4] = o.ZX;
+  fin8[5] = o.Z2;
+  fin8[6] = o.dZ;
+  fin8[7] = o.dZ2;
+}
+
+// per8 (PER8, flat segment only): [8][M] field-major; fin: 8 doubles per segment (may be NULL)
+template <bool PER8, class Seg>
+__global__ void __launch_bounds__(EV_THREADS) k_ev_apply(Seg seg, int64_t T, const EvMap* tile_excl,
+                                                         const EvMap* thr_excl, double* per8, double* fin) {
+  __shared__ double s_stage[PER8 ? EV_TILE : 1];
+  const int r = blockIdx.y;
+  const int64_t M = seg.len(r), stop = seg.stop_at(r);
+  const int64_t tile = blockIdx.x;
+  const int tid = threadIdx.x, l = lane_id(), w = tid / 64;
+  const double ninf = -__builtin_inf();
   const EvState s_init = {ninf, 0.0, 0.0, ninf, ninf, ninf, ninf, ninf};
-  s = evmap_apply(pre, s_init);
+  double* fin8 = fin ? fin + 8 * (int64_t)r : nullptr;
+  if (M == 0) {
+    if (fin8 && tile == 0 && tid == 0) write_fin8(fin8, s_init);
+    return;
+  }
+  if (tile * EV_TILE >= M) return;  // block-uniform
+  const auto src = seg.src(r);
+  const int64_t base = tile * EV_TILE + (int64_t)tid * EV_ITEMS;
+  const EvMap pre = evmap_then(tile_excl[(int64_t)r * T + tile], thr_excl[((int64_t)r * T + tile) * EV_THREADS + tid]);
+  EvState s = evmap_apply(pre, s_init);
   s.L = (base == 0) ? ninf : (base - 1 < stop ? src.L(base - 1) : ninf);
   s.dZ = ninf;
+  const int64_t last = (stop < M ? stop : M) - 1;
   EvState out[PER8 ? EV_ITEMS : 1];
 #pragma unroll
   for (int k = 0; k < EV_ITEMS; ++k) {
-    int64_t i = base + k;
+    const int64_t i = base + k;
     EvState o = s_init;
     if (i < M && i < stop) {
-      o = evmap_apply(item[k], s);
-      o.L = item_L[k];
-      o.dZ = item_lt[k] + s.X;
+      o = ev_step(s, src.N(i), src.L(i));
       s = o;
     }
     if constexpr (PER8) out[k] = o;
-    if (fin8 && i < M && i == (stop < M ? stop : M) - 1) {
-      fin8[0] = o.L;
-      fin8[1] = o.X;
-      fin8[2] = o.X2;
-      fin8[3] = o.Z;
-      fin8[4] = o.ZX;
-      fin8[5] = o.Z2;
-      fin8[6] = o.dZ;
-      fin8[7] = o.dZ2;
-    }
+    if (fin8 && i == last) write_fin8(fin8, o);
   }
   if constexpr (PER8) {
     // field by field through a per-wavefront LDS stage, so that each store writes 64 consecutive
@@ -228,11 +237,26 @@ __device__ __forceinline__ void evidence_scan_tile(const Src& src, int64_t M, in
   }
 }
 
-template <bool PER8>
-__global__ void __launch_bounds__(EV_THREADS) k_evidence_scan(const double* logL, const double* nlive, int64_t M,
-                                                              int64_t stop, double* per8, double* fin8,
-                                                              TileStatus* status, uint32_t* tile_counter) {
-  evidence_scan_tile<PER8>(EvSrcFlat{logL, nlive}, M, stop, per8, fin8, status, tile_counter);
+// workspace of the scan of R segments of at most Mmax samples: tile aggregates / prefixes, then the
+// threads' prefixes within their tiles
+static int64_t ev_tiles(int64_t Mmax) { return Mmax > 0 ? (Mmax + EV_TILE - 1) / EV_TILE : 1; }
+static size_t ev_ws_bytes(int R, int64_t Mmax) {
+  const int64_t T = ev_tiles(Mmax);
+  return (size_t)R * T * sizeof(EvMap) * (1 + EV_THREADS) + 256;
+}
+
+template <class Seg>
+static void ev_launch(const Seg& seg, int R, int64_t Mmax, void* ws, double* per8, double* fin, hipStream_t s) {
+  const int64_t T = ev_tiles(Mmax);
+  EvMap* tile_agg = (EvMap*)ws;
+  EvMap* thr_excl = tile_agg + (size_t)R * T;
+  const dim3 grid((unsigned)T, (unsigned)R);
+  hipLaunchKernelGGL(k_ev_reduce<Seg>, grid, dim3(EV_THREADS), 0, s, seg, T, tile_agg, thr_excl);
+  hipLaunchKernelGGL(k_ev_scan_tiles<Seg>, dim3(1, (unsigned)R), dim3(EV_SCAN_THREADS), 0, s, seg, T, tile_agg);
+  if (per8)
+    hipLaunchKernelGGL((k_ev_apply<true, Seg>), grid, dim3(EV_THREADS), 0, s, seg, T, tile_agg, thr_excl, per8, fin);
+  else
+    hipLaunchKernelGGL((k_ev_apply<false, Seg>), grid, dim3(EV_THREADS), 0, s, seg, T, tile_agg, thr_excl, per8, fin);
 }
 
 __global__ void k_fin_init(double* fin8) {
@@ -241,28 +265,15 @@ __global__ void k_fin_init(double* fin8) {
   for (int i = 0; i < 8; ++i) fin8[i] = v[i];
 }
 
-size_t evidence_scan_ws(int64_t M) {
-  int64_t tiles = (M + EV_TILE - 1) / EV_TILE;
-  if (tiles < 1) tiles = 1;
-  return 256 + (size_t)tiles * sizeof(TileStatus);
-}
+size_t evidence_scan_ws(int64_t M) { return ev_ws_bytes(1, M); }
 
 hipError_t evidence_scan(const double* logL, const double* n, int64_t M, int64_t stop, double* per8, double* fin8,
                          void* ws, size_t ws_bytes, hipStream_t s) {
+  (void)ws_bytes;
   if (stop < 0 || stop > M) stop = M;
-  int64_t tiles = (M + EV_TILE - 1) / EV_TILE;
   if (fin8 && (stop == 0 || M == 0)) hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1), 0, s, fin8);
   if (M == 0) return hipGetLastError();
-  uint32_t* counter = (uint32_t*)ws;
-  TileStatus* status = (TileStatus*)((char*)ws + 256);
-  int64_t words = (256 + tiles * (int64_t)sizeof(TileStatus)) / 4;
-  hipLaunchKernelGGL(k_fill_zero_u32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, (uint32_t*)ws, words);
-  if (per8)
-    hipLaunchKernelGGL(k_evidence_scan<true>, dim3((unsigned)tiles), dim3(EV_THREADS), 0, s, logL, n, M, stop, per8,
-                       stop > 0 ? fin8 : nullptr, status, counter);
-  else
-    hipLaunchKernelGGL(k_evidence_scan<false>, dim3((unsigned)tiles), dim3(EV_THREADS), 0, s, logL, n, M, stop,
-                       per8, stop > 0 ? fin8 : nullptr, status, counter);
+  ev_launch(EvSegFlat{logL, n, M, stop}, 1, M, ws, per8, stop > 0 ? fin8 : nullptr, s);
   return hipGetLastError();
 }
 
@@ -335,11 +346,6 @@ struct SegFixed {
   int64_t n;
   __device__ int64_t len(int) const { return n; }
 };
-
-__device__ __forceinline__ int64_t batch_M(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int r) {
-  const int64_t ns = states[r].num_samples, ms = runs[r].max_samples;
-  return ns < ms ? ns : ms;
-}
 
 struct SegBatchNodes {
   const hipns_run_desc_t* runs;
@@ -860,54 +866,30 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_b_scan_apply(const hipns_run_d
   }
 }
 
-__global__ void __launch_bounds__(EV_THREADS) k_b_evidence_scan(const hipns_run_desc_t* runs,
-                                                                const hipns_run_state_t* states, const double* samp_logL,
-                                                                int64_t S, const int32_t* perm, const int64_t* num_live,
-                                                                TileStatus* status, int64_t Tev, uint32_t* counters,
-                                                                double* fin) {
-  const int r = blockIdx.y;
-  const int64_t M = batch_M(runs, states, r);
-  double* fin8 = fin + 8 * r;
-  if (M == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      const double ninf = -__builtin_inf();
-      const double v[8] = {ninf, 0.0, 0.0, ninf, ninf, ninf, ninf, ninf};
-      for (int i = 0; i < 8; ++i) fin8[i] = v[i];
-    }
-    return;
-  }
-  const EvSrcSorted src{samp_logL + runs[r].samp_off, perm + r * S, num_live + r * S};
-  evidence_scan_tile<false>(src, M, M, nullptr, fin8, status + r * Tev, counters + r);
-}
-
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 struct BatchEvLayout {
-  int64_t S, Tscan, Tev;
+  int64_t S, Tscan;
   size_t zero_bytes, total;
-  size_t off_status, off_counters, off_perm, off_radix, off_nl, off_ts;
+  size_t off_perm, off_radix, off_nl, off_ts;
 };
 
 static BatchEvLayout batch_ev_layout(int R, int64_t max_M) {
   BatchEvLayout L{};
   L.S = round_up(max_M + 1, RS_TILE);
   L.Tscan = (L.S + SCAN_TILE - 1) / SCAN_TILE;
-  L.Tev = (L.S + EV_TILE - 1) / EV_TILE;
   size_t o = 0;
   o += (size_t)R * L.S * sizeof(int32_t);  // deg
-  o = round_up(o, 256);
-  L.off_status = o;
-  o += (size_t)R * L.Tev * sizeof(TileStatus);
-  o = round_up(o, 256);
-  L.off_counters = o;
-  o += (size_t)R * sizeof(uint32_t);
   o = round_up(o, 256);
   L.zero_bytes = o;
   L.off_perm = o;
   o += (size_t)R * L.S * sizeof(int32_t);
   o = round_up(o, 256);
-  L.off_radix = o;  // the node keys are written into the radix sort's first key buffer
-  o += radix_ws_bytes(R, L.S);
+  // the node keys go into the radix sort's first key buffer; once the sort is done the region holds
+  // the evidence scan's tile and thread prefixes
+  L.off_radix = o;
+  const size_t rb = radix_ws_bytes(R, L.S), eb = ev_ws_bytes(R, L.S);
+  o += rb > eb ? rb : eb;
   o = round_up(o, 256);
   L.off_nl = o;
   o += (size_t)R * L.S * sizeof(int64_t);
@@ -924,8 +906,6 @@ hipError_t batch_evidence(const hipns_run_desc_t* runs, const hipns_run_state_t*
   const BatchEvLayout L = batch_ev_layout(R, max_M);
   char* base = (char*)ws;
   int32_t* deg = (int32_t*)base;
-  TileStatus* status = (TileStatus*)(base + L.off_status);
-  uint32_t* counters = (uint32_t*)(base + L.off_counters);
   int32_t* perm = (int32_t*)(base + L.off_perm);
   const RadixWs rw = radix_ws(base + L.off_radix, R, L.S);
   double* nodes = (double*)rw.kA;
@@ -947,8 +927,7 @@ hipError_t batch_evidence(const hipns_run_desc_t* runs, const hipns_run_state_t*
   hipLaunchKernelGGL(k_b_scan_sums, dim3(1, R), dim3(1024), 0, s, ts, L.Tscan);
   hipLaunchKernelGGL(k_b_scan_apply, dim3((unsigned)L.Tscan, R), dim3(SCAN_THREADS), 0, s, runs, states, deg, perm,
                      L.S, L.Tscan, ts, nl);
-  hipLaunchKernelGGL(k_b_evidence_scan, dim3((unsigned)L.Tev, R), dim3(EV_THREADS), 0, s, runs, states, B.samp_logL,
-                     L.S, perm, nl, status, L.Tev, counters, fin);
+  ev_launch(EvSegBatch{runs, states, B.samp_logL, perm, nl, L.S}, R, L.S, base + L.off_radix, nullptr, fin, s);
   return hipGetLastError();
 }
 
