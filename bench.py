@@ -482,7 +482,8 @@ def main():
     ap.add_argument("--to-results-scale", type=int, default=0,
                     help="time the evidence scan, count_crossed_edges and the stable argsort at this many samples")
     ap.add_argument("--copy-calib", type=int, default=0,
-                    help="with --profile-pass: also copy a buffer of this many MiB (counter calibration)")
+                    help="with --profile-pass / --to-results-scale: also copy a buffer of this many MiB first "
+                         "(counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this build has one GPU; "
@@ -490,6 +491,15 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.copy_calib and (args.profile_pass or args.to_results_scale):
+        n = args.copy_calib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log(f"copy calibration: 3 x {args.copy_calib} MiB read + written")
     if args.to_results_scale:
         print(json.dumps(to_results_scale(args.to_results_scale, args.steps)), flush=True)
         return
@@ -519,14 +529,6 @@ def main():
             "log_Z_true": float(wl.log_Z), "wall_s": wall, "roofline": roof, "cpu_baseline": None}), flush=True)
         return
     if args.profile_pass:
-        if args.copy_calib:
-            n = args.copy_calib * (1 << 20) // 4
-            a = torch.ones(n, dtype=torch.float32, device="cuda")
-            b = torch.empty_like(a)
-            for _ in range(3):
-                b.copy_(a)
-            torch.cuda.synchronize()
-            log(f"copy calibration: 3 x {args.copy_calib} MiB read + written")
         durs, work, evs = instrumented_slice_timing(wl, R, args.lpc, args.wpb, seed0=90000)
         print(json.dumps({"profile_pass": True, "config": args.config, "runs": R, "lanes_per_chain": args.lpc,
                           "waves_per_block": args.wpb, "sample_launches": int(durs.size),
