@@ -260,7 +260,7 @@ int hipns_init_prepare(const hipns_run_desc_t* runs_dev, hipns_run_state_t* stat
   for (int r = 0; r < R; ++r) max_ms = runs_host[r].max_samples > max_ms ? runs_host[r].max_samples : max_ms;
   hipLaunchKernelGGL(hipns::k_init_state, dim3((R + 63) / 64), dim3(64), 0, s, states_dev, R);
   HIPNS_CHECK_LAUNCH("k_init_state");
-  hipLaunchKernelGGL(hipns::k_init_samples, dim3(rows_blocks(max_ms, bufs->dps), R), dim3(256), 0, s, runs_dev, R,
+  hipLaunchKernelGGL(hipns::k_init_samples, dim3(rows_blocks(max_ms, 1), R), dim3(256), 0, s, runs_dev, R,
                      *bufs);
   HIPNS_CHECK_LAUNCH("k_init_samples");
   return HIPNS_OK;
@@ -330,7 +330,7 @@ int hipns_shell_discard(const hipns_run_desc_t* runs_dev, hipns_run_state_t* sta
                         const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, void* stream) {
   int rc, maxD, maxN, maxS;
   if ((rc = shell_common(runs_dev, states_dev, R, runs_host, bufs, nullptr, &maxD, &maxN, &maxS))) return rc;
-  hipLaunchKernelGGL(hipns::k_discard, dim3(rows_blocks(maxS, bufs->dps), R), dim3(256), 0, as_stream(stream),
+  hipLaunchKernelGGL(hipns::k_discard, dim3(rows_blocks(maxS, bufs->dps / 4), R), dim3(256), 0, as_stream(stream),
                      runs_dev, states_dev, *bufs);
   HIPNS_CHECK_LAUNCH("k_discard");
   return HIPNS_OK;
@@ -399,7 +399,7 @@ int hipns_consume_live(const hipns_run_desc_t* runs_dev, hipns_run_state_t* stat
   int maxD, maxN, maxS;
   if ((rc = check_runs(runs_host, R, bufs, &maxD, &maxN, &maxS))) return rc;
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(hipns::k_consume, dim3(rows_blocks(maxN, bufs->dps), R), dim3(256), 0, s, runs_dev, states_dev,
+  hipLaunchKernelGGL(hipns::k_consume, dim3(rows_blocks(maxN, bufs->dps / 4), R), dim3(256), 0, s, runs_dev, states_dev,
                      *bufs);
   HIPNS_CHECK_LAUNCH("k_consume");
   hipLaunchKernelGGL(hipns::k_consume_advance, dim3((R + 63) / 64), dim3(64), 0, s, runs_dev, states_dev, R);

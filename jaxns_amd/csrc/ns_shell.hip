@@ -23,13 +23,18 @@ __global__ void __launch_bounds__(256) k_discard(const hipns_run_desc_t* runs, h
   const hipns_run_desc_t run = runs[r];
   const int S = run.shell, N = run.N, dps = B.dps, cur = st->buf;
   const int64_t next = st->next_idx;
-  const float* srcU = B.live_U[cur] + run.live_off * dps;
-  const int64_t total = (int64_t)S * dps;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = e / dps, d = e % dps;
-    int64_t j = (next + i) % run.max_samples + run.samp_off;
-    B.samp_U[j * dps + d] = srcU[e];
-    if (d == 0) {
+  // one float4 of a row per thread (dps is a multiple of 32)
+  const uint32_t q = (uint32_t)dps >> 2;
+  const float4* srcU = (const float4*)(B.live_U[cur] + run.live_off * dps);
+  float4* dstU = (float4*)B.samp_U;
+  const uint32_t total = (uint32_t)S * q;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+    const uint32_t i = e / q, c = e - i * q;
+    int64_t j = next + i;
+    if (j >= run.max_samples) j %= run.max_samples;  // _add_samples_to_state's index wrap (rare)
+    j += run.samp_off;
+    dstU[j * q + c] = srcU[e];
+    if (c == 0) {
       B.samp_logL[j] = B.live_logL[cur][run.live_off + i];
       B.samp_sender[j] = B.live_sender[cur][run.live_off + i];
       B.samp_evals[j] = B.live_evals[cur][run.live_off + i];
@@ -145,12 +150,16 @@ __global__ void __launch_bounds__(1024) k_replace(const hipns_run_desc_t* runs, 
       B.live_evals[nxt][lo_ + p] = B.live_evals[cur][lo_ + src];
     }
   }
-  const int64_t tot = (int64_t)N * dps;
-  for (int64_t e = tid; e < tot; e += nt) {
-    int p = (int)(e / dps), d = (int)(e % dps);
-    int src = perm[p];
-    float v = src < S ? B.new_U[(run.chain_off + src) * dps + d] : B.live_U[cur][(lo_ + src) * dps + d];
-    B.live_U[nxt][(lo_ + p) * dps + d] = v;
+  // rows moved one float4 per thread (dps is a multiple of 32)
+  const uint32_t q = (uint32_t)dps >> 2;
+  const float4* newU = (const float4*)B.new_U;
+  const float4* curU = (const float4*)B.live_U[cur];
+  float4* nxtU = (float4*)B.live_U[nxt];
+  const uint32_t tot = (uint32_t)N * q;
+  for (uint32_t e = tid; e < tot; e += nt) {
+    const uint32_t p = e / q, c = e - p * q;
+    const int src = perm[p];
+    nxtU[(lo_ + p) * q + c] = src < S ? newU[(run.chain_off + src) * q + c] : curU[(lo_ + src) * q + c];
   }
 }
 
@@ -373,20 +382,19 @@ __global__ void k_init_state(hipns_run_state_t* states, int R) {
 }
 
 // sample-collection defaults (initialisation.py:39-45): logL +inf, U 0.5, sender 0
+// The sample collection of create_init_state (initialisation.py:20-84): log L = +inf and sender,
+// evals and phantom 0 in every entry.  The U rows are not written: entries past num_samples are never
+// read (results take the first num_samples rows), so the reference's U placeholder has no reader.
 __global__ void k_init_samples(const hipns_run_desc_t* runs, int R, hipns_buffers_t B) {
   const int r = blockIdx.y;
   const hipns_run_desc_t run = runs[r];
-  const int dps = B.dps;
-  const int64_t total = run.max_samples * dps;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t j = e / dps + run.samp_off;
-    B.samp_U[run.samp_off * dps + e] = 0.5f;
-    if (e % dps == 0) {
-      B.samp_logL[j] = __builtin_inf();
-      B.samp_sender[j] = 0;
-      B.samp_evals[j] = 0;
-      B.samp_phantom[j] = 0;
-    }
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < run.max_samples;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = run.samp_off + e;
+    B.samp_logL[j] = __builtin_inf();
+    B.samp_sender[j] = 0;
+    B.samp_evals[j] = 0;
+    B.samp_phantom[j] = 0;
   }
 }
 
@@ -398,12 +406,17 @@ __global__ void __launch_bounds__(256) k_consume(const hipns_run_desc_t* runs, h
   const hipns_run_desc_t run = runs[r];
   const int N = run.N, dps = B.dps, cur = st->buf;
   const int64_t next = st->next_idx;
-  const int64_t total = (int64_t)N * dps;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = e / dps, d = e % dps;
-    int64_t j = (next + i) % run.max_samples + run.samp_off;
-    B.samp_U[j * dps + d] = B.live_U[cur][(run.live_off + i) * dps + d];
-    if (d == 0) {
+  const uint32_t q = (uint32_t)dps >> 2;
+  const float4* srcU = (const float4*)(B.live_U[cur] + run.live_off * dps);
+  float4* dstU = (float4*)B.samp_U;
+  const uint32_t total = (uint32_t)N * q;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+    const uint32_t i = e / q, c = e - i * q;
+    int64_t j = next + i;
+    if (j >= run.max_samples) j %= run.max_samples;
+    j += run.samp_off;
+    dstU[j * q + c] = srcU[e];
+    if (c == 0) {
       B.samp_logL[j] = B.live_logL[cur][run.live_off + i];
       B.samp_sender[j] = B.live_sender[cur][run.live_off + i];
       B.samp_evals[j] = B.live_evals[cur][run.live_off + i];
