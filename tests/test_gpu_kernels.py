@@ -126,6 +126,32 @@ def test_evidence_scan_vs_oracle(M, stop):
         np.testing.assert_allclose(a[fa], b[fb], rtol=1e-9, atol=1e-9)
 
 
+def test_evidence_scan_closed_form_large():
+    """3 000 000 samples (1465 tiles of 2048, so the tile scan composes two tiles per thread) against
+    the closed forms of a constant likelihood L = 1 with n = N throughout: log X_i = i log(N/(N+1)),
+    log X2_i = i log(N/(N+2)), and Z_M = (1 - X_1)/2 + X_1 - X_M (the initial carry has log L = -inf,
+    so the first mean likelihood is 1/2)."""
+    torch = _torch()
+    M, N = 3_000_000, 4096.0
+    say(f"evidence_scan closed form M={M}")
+    L = torch.zeros(M, dtype=torch.float64, device="cuda")
+    n = torch.full((M,), N, dtype=torch.float64, device="cuda")
+    per = torch.empty((8, M), dtype=torch.float64, device="cuda")
+    fin = torch.empty(8, dtype=torch.float64, device="cuda")
+    ws = torch.empty(lib().hipns_evidence_scan_workspace_size(M), dtype=torch.uint8, device="cuda")
+    check(lib().hipns_evidence_scan(ptr(L), ptr(n), M, -1, ptr(per), ptr(fin), ptr(ws), ws.numel(), _stream()))
+    i = np.array([1, 2, 2047, 2048, 2049, 1024 * 2048 - 1, 1024 * 2048, 2_500_001, M], dtype=np.int64)
+    logX = per[1].cpu().numpy()[i - 1]
+    logX2 = per[2].cpu().numpy()[i - 1]
+    np.testing.assert_allclose(logX, i * np.log(N / (N + 1.0)), rtol=1e-9)
+    np.testing.assert_allclose(logX2, i * np.log(N / (N + 2.0)), rtol=1e-9)
+    X1 = N / (N + 1.0)
+    Z = 0.5 * (1.0 - X1) + X1 - np.exp(M * np.log(X1))
+    f = fin.cpu().numpy()
+    np.testing.assert_allclose(f[3], np.log(Z), rtol=0, atol=1e-10)  # Z is close to 1: absolute in log Z
+    np.testing.assert_allclose(f[1], M * np.log(X1), rtol=1e-9)
+
+
 def test_argsort_stable_vs_numpy():
     torch = _torch()
     rng = np.random.default_rng(3)
