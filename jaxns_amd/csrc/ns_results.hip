@@ -1,5 +1,6 @@
 // ns_results.hip -- to_results primitives: the evidence scan over all samples,
 // a stable FP64 argsort, and count_crossed_edges.
+#include <type_traits>
 #include <utility>
 
 #include "hipns_dev.h"
@@ -360,10 +361,11 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_count(const void* kin, int64_
   __shared__ uint32_t hist[RS_BINS];
   const int r = blockIdx.y, t = blockIdx.x, tid = threadIdx.x;
   const int64_t n = seg.len(r);
+  const int64_t base = (int64_t)t * RS_TILE;
+  if (base >= n) return;  // block-uniform: the scans read only the segment's own tiles
   hist[tid] = 0u;
   __syncthreads();
-  const int64_t base = (int64_t)t * RS_TILE;
-  if (base < n) {  // block-uniform
+  {
     const uint64_t lt = (1ull << lane_id()) - 1ull;
     // all loads first (clamped in-bounds index, so they issue together), then the histogram
     uint64_t kr[RS_ITEMS];
@@ -402,6 +404,39 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scan(uint32_t* counts, int T,
     run += v;
   }
   if (threadIdx.x == 0) totals[r * RS_BINS + d] = tot;
+}
+
+// the same per segment for batches: one block per segment scans only that segment's Tr tiles (the
+// count launch writes nothing for empty tiles), staging 16 columns of all 256 rows in LDS at a time;
+// thread d scans digit d's row
+constexpr int RS_SCAN_COLS = 16;
+template <class Seg>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scan_seg(Seg seg, uint32_t* counts, int T, uint32_t* totals) {
+  __shared__ uint32_t sc[RS_BINS * RS_SCAN_COLS];
+  const int r = blockIdx.x, d = threadIdx.x;
+  const int Tr = (int)((seg.len(r) + RS_TILE - 1) / RS_TILE);
+  uint32_t* c = counts + (int64_t)r * RS_BINS * T;
+  uint32_t run = 0u;
+  for (int c0 = 0; c0 < Tr; c0 += RS_SCAN_COLS) {
+    const int w = Tr - c0 < RS_SCAN_COLS ? Tr - c0 : RS_SCAN_COLS;
+    for (int i = d; i < RS_BINS * w; i += RS_THREADS) {
+      const int row = i / w, col = i - row * w;
+      sc[i] = c[(int64_t)row * T + c0 + col];
+    }
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+      const uint32_t v = sc[d * w + j];
+      sc[d * w + j] = run;
+      run += v;
+    }
+    __syncthreads();
+    for (int i = d; i < RS_BINS * w; i += RS_THREADS) {
+      const int row = i / w, col = i - row * w;
+      c[(int64_t)row * T + c0 + col] = sc[i];
+    }
+    __syncthreads();
+  }
+  totals[r * RS_BINS + d] = run;
 }
 
 // stable scatter of one pass; the first pass reads FP64 keys (codes formed here) and takes the
@@ -561,7 +596,12 @@ static hipError_t radix_argsort(const double* keys, int R, int64_t S, Seg seg, i
       hipLaunchKernelGGL((k_rs_count<true, Seg>), grid, dim3(RS_THREADS), 0, s, kin, S, T, seg, shift, w.counts);
     else
       hipLaunchKernelGGL((k_rs_count<false, Seg>), grid, dim3(RS_THREADS), 0, s, kin, S, T, seg, shift, w.counts);
-    hipLaunchKernelGGL(k_rs_scan, dim3(RS_BINS, (unsigned)R), dim3(RS_THREADS), 0, s, w.counts, T, w.totals);
+    // the digit-parallel scan reads every tile of a row, so only for one flat array (all its tiles
+    // hold keys); a batch segment's tiles past its length hold no counts
+    if (std::is_same<Seg, SegFixed>::value && T > RS_SCAN_COLS)
+      hipLaunchKernelGGL(k_rs_scan, dim3(RS_BINS, 1), dim3(RS_THREADS), 0, s, w.counts, T, w.totals);
+    else
+      hipLaunchKernelGGL((k_rs_scan_seg<Seg>), dim3((unsigned)R), dim3(RS_THREADS), 0, s, seg, w.counts, T, w.totals);
     if (p == 0)
       hipLaunchKernelGGL((k_rs_scatter<true, false, Seg>), grid, dim3(RS_THREADS), 0, s, kin, vin, S, T, seg, shift,
                          w.counts, w.totals, kout, vout);
@@ -587,6 +627,18 @@ hipError_t argsort_stable(const double* keys, int64_t n, int32_t* perm, void* ws
 // ===========================================================================
 // count_crossed_edges (internals/tree_structure.py:33-108)
 // ===========================================================================
+// deg[node] += 1 for each lane's node (node < 0: nothing), one atomic per run of equal nodes in
+// consecutive lanes: the samples of one shell share their sender, so runs are long
+__device__ __forceinline__ void add_degree_runs(int32_t* deg, int64_t node) {
+  const int l = lane_id();
+  const int64_t prev = __shfl_up(node, 1, 64);
+  const bool start = l == 0 || node != prev;
+  const uint64_t starts = __ballot(start);
+  const uint64_t later = l == 63 ? 0ull : starts & (~0ull << (l + 1));
+  const int end = later ? __ffsll((unsigned long long)later) - 1 : 64;
+  if (start && node >= 0) atomicAdd(&deg[node], end - l);
+}
+
 __global__ void k_nodes(const double* logL, int64_t N, double* nodes) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i > N) return;
@@ -594,10 +646,13 @@ __global__ void k_nodes(const double* logL, int64_t N, double* nodes) {
 }
 
 __global__ void k_out_degree(const int32_t* sender, int64_t N, int32_t* deg) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  int32_t s = sender[i] > 0 ? sender[i] : 0;  // lax.max(sender, 0)
-  if (s <= N) atomicAdd(&deg[s], 1);           // out-of-range scatter updates are dropped
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t node = -1;
+  if (i < N) {
+    const int32_t s = sender[i] > 0 ? sender[i] : 0;  // lax.max(sender, 0)
+    if (s <= N) node = s;                             // out-of-range scatter updates are dropped
+  }
+  add_degree_runs(deg, node);
 }
 
 constexpr int SCAN_THREADS = 256;
@@ -761,20 +816,28 @@ __global__ void k_b_nodes(const hipns_run_desc_t* runs, const hipns_run_state_t*
 }
 
 // out-degree of every node by sender (tree_structure.py:33-108) and the phantom count of each run
+constexpr int OUT_DEGREE_ITEMS = 8;
+
 __global__ void __launch_bounds__(256) k_b_out_degree(const hipns_run_desc_t* runs, const hipns_run_state_t* states,
                                                       hipns_buffers_t B, int64_t S, int32_t* deg,
                                                       unsigned long long* phantom) {
   __shared__ unsigned long long red[4];
   const int r = blockIdx.y;
   const int64_t M = batch_M(runs, states, r);
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x * OUT_DEGREE_ITEMS + threadIdx.x;
   const int64_t off = runs[r].samp_off;
   unsigned long long ph = 0;
-  if (i < M) {
-    int32_t snd = B.samp_sender[off + i];
-    int64_t sn = snd > 0 ? snd : 0;                // lax.max(sender, 0)
-    if (sn <= M) atomicAdd(&deg[r * S + sn], 1);  // out-of-range scatter updates are dropped
-    ph = B.samp_phantom[off + i] ? 1ull : 0ull;
+  // OUT_DEGREE_ITEMS samples per thread, 256 apart (each wavefront covers 64 consecutive samples)
+  for (int k = 0; k < OUT_DEGREE_ITEMS; ++k) {
+    const int64_t ik = i + (int64_t)k * blockDim.x;
+    int64_t node = -1;
+    if (ik < M) {
+      int32_t snd = B.samp_sender[off + ik];
+      int64_t sn = snd > 0 ? snd : 0;  // lax.max(sender, 0)
+      if (sn <= M) node = sn;          // out-of-range scatter updates are dropped
+      ph += B.samp_phantom[off + ik] ? 1ull : 0ull;
+    }
+    add_degree_runs(deg + r * S, node);
   }
   for (int o = 32; o > 0; o >>= 1) ph += __shfl_down(ph, o, 64);
   if (lane_id() == 0) red[threadIdx.x / 64] = ph;
@@ -920,8 +983,9 @@ hipError_t batch_evidence(const hipns_run_desc_t* runs, const hipns_run_state_t*
                      L.S, nodes);
   hipError_t e = radix_argsort(nodes, R, L.S, SegBatchNodes{runs, states}, perm, rw, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_b_out_degree, dim3((unsigned)((L.S + 255) / 256), R), dim3(256), 0, s, runs, states, B, L.S,
-                     deg, (unsigned long long*)phantom);
+  const unsigned deg_blocks = (unsigned)((L.S + 256 * OUT_DEGREE_ITEMS - 1) / (256 * OUT_DEGREE_ITEMS));
+  hipLaunchKernelGGL(k_b_out_degree, dim3(deg_blocks, R), dim3(256), 0, s, runs, states, B, L.S, deg,
+                     (unsigned long long*)phantom);
   hipLaunchKernelGGL(k_b_scan_tiles, dim3((unsigned)L.Tscan, R), dim3(SCAN_THREADS), 0, s, runs, states, deg, perm,
                      L.S, L.Tscan, ts);
   hipLaunchKernelGGL(k_b_scan_sums, dim3(1, R), dim3(1024), 0, s, ts, L.Tscan);

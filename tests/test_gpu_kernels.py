@@ -509,21 +509,23 @@ def test_linreg_hyperparameter_batch_logZ():
     assert all(st.termination_reason & 4 for st in b.states())
 
 
-def test_batch_evidence_matches_per_run():
+@pytest.mark.parametrize("nruns,ms_mult", [(5, 200), (1, 400)])
+def test_batch_evidence_matches_per_run(nruns, ms_mult):
     """hipns_batch_evidence (all runs in one set of launches) against the single-run primitives
     (count_crossed_edges + evidence scan per run), on uneven runs with phantom samples (k=2), so
-    the samples are not already sorted: phantom counts exact, evidence statistics to 1e-12."""
+    the samples are not already sorted: phantom counts exact, evidence statistics to 1e-12.  The
+    one-run batch has a sort segment of more than 16 tiles of which the run fills only some."""
     import torch
     from jaxns_amd.engine import DeviceBatch, RunSpec
-    say("batched evidence vs per run")
+    say(f"batched evidence vs per run: {nruns} runs, max_samples = {ms_mult} N")
     rng = np.random.default_rng(77)
     specs = []
-    for r in range(5):
+    for r in range(nruns):
         D = int(rng.integers(2, 7))
         N = int(rng.integers(100, 300))
         model, _, _, _ = gmm_problem(D, seed=500 + r)
-        specs.append(RunSpec(model=model, num_live_points=N, num_slices=4 * D, k_phantom=2, max_samples=N * 200,
-                             seed=r))
+        specs.append(RunSpec(model=model, num_live_points=N, num_slices=4 * D, k_phantom=2,
+                             max_samples=N * ms_mult, seed=r))
     b = DeviceBatch(specs, lanes_per_chain=4, waves_per_block=2)
     b.run(dict(dlogZ=math.log(1.0 + 1e-2), max_samples=1e18), graph_iters=4)
     z1, fin1, ph1 = b.log_Z_per_run()
