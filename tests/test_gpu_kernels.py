@@ -389,10 +389,10 @@ def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb, faces):
         assert np.all(steps == 1)
 
 
-def test_full_run_config1_logZ():
+def test_full_run_mvn_d10_n300_logZ():
     """Plain single run, D=10 MVN, N=300, s=5: log Z vs analytic within 4 sigma + 0.1."""
     import jaxns_amd as J
-    say("full run config 1")
+    say("full run, MVN D=10, N=300")
     model, log_Z, _, _ = mvn_problem(10, seed=0)
     ns = J.NestedSampler(model, num_live_points=300, s=5, k=0, max_samples=300 * 200)
     reason, state = ns(1)
