@@ -39,9 +39,9 @@ PEAK_FP32_MFMA = 157.3e12
 
 # HBM bytes per k_sample launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the same
 # instrumented batch (counters scaled by a 256 MiB copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
-# profiles/k_sample_mvn_d32_n2048_r512.md and profiles/k_sample_linreg_4096x16_n1024_r256.md.
+# profiles/k_sample_mvn_d32_n2048_r1024.md and profiles/k_sample_linreg_4096x16_n1024_r256.md.
 # Keyed by (workload, runs per launch); other shapes report null.
-MEASURED_TRAFFIC = {("c2", 512): 184.93e6, ("c4", 256): 32.55e6}
+MEASURED_TRAFFIC = {("c2", 512): 184.93e6, ("c2", 1024): 369.82e6, ("c4", 256): 32.55e6}
 PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
 
@@ -467,7 +467,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--runs", type=int, default=None,
-                    help="runs per step (chosen form); default c2: 512, c1: 128, c3/c4: 256, c5: 8")
+                    help="runs per step (chosen form); default c2: 1024, c1: 128, c3/c4: 256, c5: 8")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--shells", type=int, default=3, help="c5: shells per partial run")
     ap.add_argument("--graph-iters", type=int, default=8)
@@ -506,9 +506,9 @@ def main():
     wl = make_workload(args.config)
     if args.config == "c4":
         wl.lr_terms = args.lr_terms
-    # c2: 512 runs per step (128: 793 runs/s, 256: 867, 512: 899, 1024: 921; the shell kernel's tail and
-    # the per-run kernels amortise over more runs)
-    R = args.runs or {"c2": 512, "c3": 256, "c4": 256, "c5": 8}.get(args.config, 128)
+    # c2: 1024 runs per step (512: 920 runs/s, 768: 939, 1024: 952 on one box; the shell kernel's tail
+    # and the per-run kernels amortise over more runs)
+    R = args.runs or {"c2": 1024, "c3": 256, "c4": 256, "c5": 8}.get(args.config, 128)
     seed_base = 0
     if args.config == "c5":
         t0 = time.perf_counter()
