@@ -11,7 +11,7 @@ shell) is timed beside it.
 
 Other workloads:
   c1  MVN D=10, num_live_points=300, s=5
-  c3  runs of uneven D in {4..64} and num_live_points in {128..1024} (512 per step by
+  c3  runs of uneven D in {4..64} and num_live_points in {128..1024} (1024 per step by
       default), 4-component isotropic Gaussian mixture, prior U[-10, 10]^D
   c4  Bayesian linear regression, 4096 rows x 16 features, sigma=0.3,
       w ~ N(0, I), num_live_points=1024, s=5; residual GEMM in BF16x3 MFMA
@@ -467,7 +467,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--runs", type=int, default=None,
-                    help="runs per step (chosen form); default c2: 1024, c1: 128, c3: 512, c4: 256, c5: 8")
+                    help="runs per step (chosen form); default c2: 1024, c1: 128, c3: 1024, c4: 256, c5: 8")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--shells", type=int, default=3, help="c5: shells per partial run")
     ap.add_argument("--graph-iters", type=int, default=8)
@@ -508,8 +508,9 @@ def main():
         wl.lr_terms = args.lr_terms
     # c2: 1024 runs per step (512: 920 runs/s, 768: 939, 1024: 952 on one box; the shell kernel's tail
     # and the per-run kernels amortise over more runs)
-    # c3: 512 (256: 54.7 runs/s, 512: 65.9; the lock-step shells' tail of the longest runs amortises)
-    R = args.runs or {"c2": 1024, "c3": 512, "c4": 256, "c5": 8}.get(args.config, 128)
+    # c3: 1024 (256: 54.7 runs/s, 512: 65.9, 1024: 74.9; the lock-step shells' tail of the longest runs
+    # amortises over more runs)
+    R = args.runs or {"c2": 1024, "c3": 1024, "c4": 256, "c5": 8}.get(args.config, 128)
     seed_base = 0
     if args.config == "c5":
         t0 = time.perf_counter()
