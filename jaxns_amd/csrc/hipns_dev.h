@@ -314,27 +314,54 @@ __device__ __forceinline__ EvMap evmap_identity() {
 
 // the single-step map of sample (n, logL) preceded by log-likelihood Lprev; the
 // per-factor expressions follow shrinkage_statistics.py:47-72.
-__device__ __forceinline__ EvMap evmap_step(double n, double Lprev, double L, double* log_tm) {
+// the coefficients of one step that depend on the live-point count n alone (T, T2, t, t2, tT)
+struct EvCoef {
+  double n, T, T2, t, t2, tT;
+};
+__device__ __forceinline__ EvCoef evcoef(double n) {
+  const double LOG2 = 0.69314718055994530942;
+  const double log_n = log(n);
+  const double log_n1 = log(n + 1.0);
+  const double log_n2 = log(n + 2.0);
+  EvCoef c;
+  c.n = n;
+  c.T = -lae(0.0, -log_n);
+  c.t = -log_n1;
+  c.T2 = -lae(0.0, LOG2 - log_n);
+  c.t2 = LOG2 - log_n1 - log_n2;
+  c.tT = c.T - log_n2;
+  return c;
+}
+// the coefficients of n, formed again only when n differs from the cached step's (runs of equal n
+// are the rule: a shell's discarded samples all have n = N); the values are those of evcoef(n)
+__device__ __forceinline__ const EvCoef& evcoef_cached(EvCoef& c, double n) {
+  if (!(c.n == n)) c = evcoef(n);
+  return c;
+}
+__device__ __forceinline__ EvCoef evcoef_empty() {
+  EvCoef c;
+  c.n = __builtin_nan("");
+  c.T = c.T2 = c.t = c.t2 = c.tT = 0.0;
+  return c;
+}
+
+__device__ __forceinline__ EvMap evmap_step_c(const EvCoef& c, double Lprev, double L, double* log_tm) {
   const double LOG2 = 0.69314718055994530942;
   double midL = -LOG2 + lae(L, Lprev);
-  double log_n = log(n);
-  double log_n1 = log(n + 1.0);
-  double log_n2 = log(n + 2.0);
-  double T = -lae(0.0, -log_n);
-  double t = -log_n1;
-  double T2 = -lae(0.0, LOG2 - log_n);
-  double t2 = LOG2 - log_n1 - log_n2;
-  double tT = T - log_n2;
   EvMap m;
-  m.a = T;
-  m.b = T2;
-  m.g = t + midL;
-  m.e = tT + midL;
-  m.z = LOG2 + t + midL;
-  m.h = t2 + 2.0 * midL;
+  m.a = c.T;
+  m.b = c.T2;
+  m.g = c.t + midL;
+  m.e = c.tT + midL;
+  m.z = LOG2 + c.t + midL;
+  m.h = c.t2 + 2.0 * midL;
   m.th = m.h;
   *log_tm = m.g;
   return m;
+}
+
+__device__ __forceinline__ EvMap evmap_step(double n, double Lprev, double L, double* log_tm) {
+  return evmap_step_c(evcoef(n), Lprev, L, log_tm);
 }
 
 // f then g
@@ -364,6 +391,14 @@ __device__ __forceinline__ EvState evmap_apply(const EvMap& f, const EvState& s)
 }
 
 // one sequential recurrence step (the reference op itself); returns the new state
+__device__ __forceinline__ EvState ev_step_c(const EvState& s, const EvCoef& c, double L) {
+  double log_tm;
+  EvMap f = evmap_step_c(c, s.L, L, &log_tm);
+  EvState o = evmap_apply(f, s);
+  o.L = L;
+  o.dZ = log_tm + s.X;
+  return o;
+}
 __device__ __forceinline__ EvState ev_step(const EvState& s, double n, double L) {
   double log_tm;
   EvMap f = evmap_step(n, s.L, L, &log_tm);

@@ -101,12 +101,13 @@ __global__ void __launch_bounds__(EV_THREADS) k_ev_reduce(Seg seg, int64_t T, Ev
   const int64_t base = tile * EV_TILE + (int64_t)tid * EV_ITEMS;
   const double ninf = -__builtin_inf();
   EvMap m = evmap_identity();
+  EvCoef cc = evcoef_empty();
   for (int k = 0; k < EV_ITEMS; ++k) {
     const int64_t i = base + k;
     if (i < M && i < stop) {
       double lt;
       const double Lp = i == 0 ? ninf : src.L(i - 1);
-      m = evmap_then(m, evmap_step(src.N(i), Lp, src.L(i), &lt));
+      m = evmap_then(m, evmap_step_c(evcoef_cached(cc, src.N(i)), Lp, src.L(i), &lt));
     }
   }
   const EvMap incl = wave_incl_scan(m);
@@ -202,12 +203,13 @@ __global__ void __launch_bounds__(EV_THREADS) k_ev_apply(Seg seg, int64_t T, con
   s.dZ = ninf;
   const int64_t last = (stop < M ? stop : M) - 1;
   EvState out[PER8 ? EV_ITEMS : 1];
+  EvCoef cc = evcoef_empty();
 #pragma unroll
   for (int k = 0; k < EV_ITEMS; ++k) {
     const int64_t i = base + k;
     EvState o = s_init;
     if (i < M && i < stop) {
-      o = ev_step(s, src.N(i), src.L(i));
+      o = ev_step_c(s, evcoef_cached(cc, src.N(i)), src.L(i));
       s = o;
     }
     if constexpr (PER8) out[k] = o;

@@ -188,11 +188,12 @@ __device__ EvMap thread_map(const SeqSpec& q, int tt, int nthreads) {
   const int b = tt * per;
   const int e = min(q.n, b + per);
   EvMap m = evmap_identity();
+  EvCoef cc = evcoef_empty();
   for (int i = b; i < e; ++i) {
     double Lp = i == 0 ? q.L0 : q.L[i - 1];
     double ni = q.nconst > 0.0 ? q.nconst : (double)(q.ndesc - i);
     double lt;
-    m = evmap_then(m, evmap_step(ni, Lp, q.L[i], &lt));
+    m = evmap_then(m, evmap_step_c(evcoef_cached(cc, ni), Lp, q.L[i], &lt));
   }
   return m;
 }
