@@ -208,6 +208,25 @@ def run_batch(batch, graph_iters):
     return batch.log_Z()
 
 
+def make_groups(wl, seeds, lpc, wpb, groups):
+    """The runs of one step split into `groups` batches (contiguous seed ranges)."""
+    seeds = list(seeds)
+    cut = np.linspace(0, len(seeds), groups + 1).round().astype(int)
+    out = [make_batch(wl, seeds[cut[g]:cut[g + 1]], lpc, wpb) for g in range(groups)]
+    return [b for b, _ in out], np.concatenate([t for _, t in out])
+
+
+def run_groups(batches, graph_iters):
+    """The chosen form with several groups: every group on its own stream, shells of one group running
+    while another's drain (engine.run_concurrent); log Z of every run, in seed order."""
+    from jaxns_amd.engine import run_concurrent
+    if len(batches) == 1:
+        batches[0].run(term_for(batches[0]), init_efficiency_threshold=0.1, graph_iters=graph_iters)
+    else:
+        run_concurrent(batches, [term_for(b) for b in batches], init_efficiency_threshold=0.1,
+                       graph_iters=graph_iters)
+
+
 def instrumented_slice_timing(wl, R, lpc, wpb, seed0):
     """Runs one batch shell by shell with HIP events around the sampling kernel
     (hipns_shell_sample) on the launch stream; returns per-launch durations and
@@ -471,6 +490,9 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--shells", type=int, default=3, help="c5: shells per partial run")
     ap.add_argument("--graph-iters", type=int, default=8)
+    ap.add_argument("--groups", type=int, default=0,
+                    help="chosen form: the runs of a step in this many concurrent groups, each on its own "
+                         "stream (default c2 and c3: 2, others 1)")
     ap.add_argument("--lpc", type=int, default=4)
     ap.add_argument("--wpb", type=int, default=4)
     ap.add_argument("--lr-terms", type=int, default=3, choices=[3, 4])
@@ -511,6 +533,9 @@ def main():
     # c3: 1024 (256: 54.7 runs/s, 512: 65.9, 1024: 74.9; the lock-step shells' tail of the longest runs
     # amortises over more runs)
     R = args.runs or {"c2": 1024, "c3": 1024, "c4": 256, "c5": 8}.get(args.config, 128)
+    # two groups on their own streams: c2 975 against 953 runs/s, c3 81.0 against 74.9 (four: 81.5)
+    G = max(1, args.groups if args.groups else {"c2": 2, "c3": 2}.get(args.config, 1))
+    RG = (R + G - 1) // G  # runs per group: the batch one launch of the dominant kernel covers
     seed_base = 0
     if args.config == "c5":
         t0 = time.perf_counter()
@@ -541,37 +566,39 @@ def main():
     if not args.no_autotune:
         from jaxns_amd.autotune import autotune
         t0 = time.perf_counter()
-        specs0, _ = wl.specs(range(seed_base, seed_base + R))
+        specs0, _ = wl.specs(range(seed_base, seed_base + RG))
         args.lpc, args.wpb = autotune(specs0, verbose=True, lr_terms=wl.lr_terms)
         log(f"autotune picked lanes_per_chain={args.lpc} waves_per_block={args.wpb} "
             f"in {time.perf_counter() - t0:.1f} s")
 
     # ---------------- chosen form: R runs per step
     for w in range(args.warmup):
-        b, _ = make_batch(wl, range(1000 + w * R, 1000 + (w + 1) * R), args.lpc, args.wpb)
+        bs, _ = make_groups(wl, range(1000 + w * R, 1000 + (w + 1) * R), args.lpc, args.wpb, G)
         t0 = time.perf_counter()
-        run_batch(b, args.graph_iters)
-        log(f"warmup {w}: {R} runs in {time.perf_counter() - t0:.2f} s")
-        del b
+        run_groups(bs, args.graph_iters)
+        for b in bs:
+            b.log_Z()
+        log(f"warmup {w}: {R} runs in {G} group(s) in {time.perf_counter() - t0:.2f} s")
+        del bs
     torch.cuda.synchronize()
     logZ, truths = [], []
     evals_total = 0
     t_total = 0.0
     for k in range(args.steps):
-        b, tr = make_batch(wl, range(seed_base + k * R, seed_base + (k + 1) * R), args.lpc, args.wpb)
+        bs, tr = make_groups(wl, range(seed_base + k * R, seed_base + (k + 1) * R), args.lpc, args.wpb, G)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        b.run(term_for(b), init_efficiency_threshold=0.1, graph_iters=args.graph_iters)
+        run_groups(bs, args.graph_iters)
         t1 = time.perf_counter()
-        z = b.log_Z()
+        z = np.concatenate([b.log_Z() for b in bs])
         torch.cuda.synchronize()
         t_total += time.perf_counter() - t0
         log(f"step {k}: sampling {t1 - t0:.3f} s, log Z of {R} runs {time.perf_counter() - t1:.3f} s")
         logZ.append(z)
         truths.append(tr)
-        evals_total += sum(st.num_likelihood_evaluations for st in b.states())
+        evals_total += sum(st.num_likelihood_evaluations for b in bs for st in b.states())
         log(f"step {k}: {R} runs, total {t_total:.2f} s, mean |dlogZ| {np.abs(z[:, 0] - tr).mean():.3f}")
-        del b
+        del bs
     logZ = np.concatenate(logZ)
     truths = np.concatenate(truths)
     ms_per_step = 1e3 * t_total / args.steps
@@ -594,14 +621,15 @@ def main():
         del b
     plain_runs_per_s = args.plain_runs / tp
 
-    # ---------------- roofline of the dominant kernel (hipns_shell_sample)
-    durs, work, evs = instrumented_slice_timing(wl, R, args.lpc, args.wpb, seed0=90000)
+    # ---------------- roofline of the dominant kernel (hipns_shell_sample), one launch covering the runs of
+    # one group (the batch size each launch of the timed chosen form has)
+    durs, work, evs = instrumented_slice_timing(wl, RG, args.lpc, args.wpb, seed0=90000)
     log(f"instrumented: {durs.size} sample launches, mean {durs.mean() * 1e3:.3f} ms")
     avg_dur = float(durs.mean())
     achieved = float(work.mean() / avg_dur)
     scale = 1e12 if wl.unit == "TFLOP/s" else 1e9
     roofline = dict(bound=wl.bound, achieved=achieved / scale, peak=wl.peak / scale, unit=wl.unit,
-                    frac=achieved / wl.peak, traffic=MEASURED_TRAFFIC.get((args.config, R)),
+                    frac=achieved / wl.peak, traffic=MEASURED_TRAFFIC.get((args.config, RG)),
                     traffic_unit="bytes per launch", kernel=wl.kernel, avg_launch_ms=avg_dur * 1e3,
                     launches=int(durs.size), evals_per_launch=float(evs.mean()),
                     evals_per_s_in_kernel=float(evs.sum() / durs.sum()),
@@ -625,7 +653,7 @@ def main():
         "vs_baseline": None,
         "dtype": wl.dtype,
         "data": "synthetic (seeded numpy)",
-        "config": {"workload": wl.name, "runs_per_step": R, "graph_iters": args.graph_iters,
+        "config": {"workload": wl.name, "runs_per_step": R, "groups": G, "graph_iters": args.graph_iters,
                    "lanes_per_chain": args.lpc, "waves_per_block": args.wpb},
         "likelihood_evals_per_s": evals_per_s,
         "evals_per_run": evals_per_run,

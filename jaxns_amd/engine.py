@@ -543,6 +543,73 @@ class DeviceBatch:
             log_efficiency=float(np.log(M) - np.log(total_evals)), termination_reason=core["termination_reason"])
 
 
+def run_concurrent(batches: List["DeviceBatch"], term, *, init_efficiency_threshold: float = 0.1,
+                   graph_iters: int = 8, max_iterations: int = 1_000_000) -> None:
+    """DeviceBatch.run for several batches at once, each on a stream of its own (default priority).
+    The uniform-sampler phase goes shell by shell and then each batch's captured graph of `graph_iters`
+    shells is replayed on its stream; in both the host cycles through the batches, reading one batch's
+    done flags (a wait on that stream only) while the others' shells run, so one batch's launches fill
+    the GPU while another's drain.  Every batch performs the same shells, in the same
+    order, as batch.run(term, graph_iters=graph_iters) would: its results are identical.  `term` is
+    one termination dict for all batches or a list of one per batch."""
+    torch = _torch()
+    terms = list(term) if isinstance(term, (list, tuple)) else [term] * len(batches)
+    if any(b.external for b in batches):
+        raise ValueError("run_concurrent: batches with a callable likelihood step on the host; use run()")
+    if graph_iters <= 0:
+        raise ValueError("run_concurrent needs graph_iters > 0")
+    streams = [torch.cuda.Stream() for _ in batches]
+    uterms = [dict(efficiency_threshold=float(init_efficiency_threshold), dlogZ=0.0,
+                   max_samples=float(min(sp.max_samples for sp in b.specs))) for b in batches]
+    for b, s, ut in zip(batches, streams, uterms):
+        with torch.cuda.stream(s):
+            b.init_state()
+            if init_efficiency_threshold > 0:
+                b.check_termination(ut, True)
+    # the uniform-sampler phase shell by shell, the batches in turn (one batch's shell runs while the
+    # host waits for another's done flags)
+    live = []
+    if init_efficiency_threshold > 0:
+        for i, (b, s) in enumerate(zip(batches, streams)):
+            with torch.cuda.stream(s):
+                if not np.all(b.done_flags()):
+                    b.set_sampler(1)
+                    live.append(i)
+    its = [0] * len(batches)
+    while live:
+        nxt = []
+        for i in live:
+            b = batches[i]
+            with torch.cuda.stream(streams[i]):
+                if its[i] >= max_iterations or np.all(b.done_flags()):
+                    b.set_sampler(0)
+                    continue
+                b.collect_shell(uterms[i])
+            its[i] += 1
+            nxt.append(i)
+        live = nxt
+    for b, s, term in zip(batches, streams, terms):
+        with torch.cuda.stream(s):
+            b.check_termination(term, True)
+    graphs = [b._get_graph(t, graph_iters) for b, t in zip(batches, terms)]  # each capture synchronises
+    its = [0] * len(batches)
+    live = list(range(len(batches)))
+    while live:
+        nxt = []
+        for i in live:
+            with torch.cuda.stream(streams[i]):
+                if its[i] >= max_iterations or np.all(batches[i].done_flags()):
+                    continue
+                graphs[i].replay()
+            its[i] += graph_iters
+            nxt.append(i)
+        live = nxt
+    for b, s in zip(batches, streams):
+        with torch.cuda.stream(s):
+            b.consume()
+    torch.cuda.synchronize()
+
+
 def _linear_to_log_stats(log_f_mean, log_f2_mean):
     """internals/stats.py:55-74."""
     fsq = log_f_mean + log_f_mean

@@ -420,6 +420,35 @@ def test_batched_runs_match_single_runs():
     assert a.log_Z_mean == b.log_Z_mean
 
 
+def test_concurrent_batches_match_sequential():
+    """engine.run_concurrent (two batches on streams of their own, their shell graphs replayed in turn)
+    gives every run the same samples, evaluation counts and log Z as DeviceBatch.run of each batch
+    alone."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec, run_concurrent
+    say("concurrent batches vs sequential")
+    model, _, _, _ = mvn_problem(8, seed=3)
+
+    def make(seeds):
+        specs = [RunSpec(model=model, num_live_points=128, num_slices=16, k_phantom=0, max_samples=128 * 100,
+                         seed=s) for s in seeds]
+        return DeviceBatch(specs, lanes_per_chain=4, waves_per_block=2)
+
+    term = dict(dlogZ=math.log(1.0 + 1e-3), max_samples=1e18)
+    con = [make([1, 2, 3]), make([4, 5])]
+    run_concurrent(con, term, graph_iters=4)
+    seq = [make([1, 2, 3]), make([4, 5])]
+    for b in seq:
+        b.run(term, graph_iters=4)
+    for a, b in zip(con, seq):
+        assert np.array_equal(a.log_Z(), b.log_Z())
+        for r in range(a.R):
+            ra, rb = a.to_results(r), b.to_results(r)
+            assert ra.total_num_samples == rb.total_num_samples
+            assert np.array_equal(ra.log_L_samples, rb.log_L_samples)
+            assert np.array_equal(ra.U_samples, rb.U_samples)
+            assert ra.total_num_likelihood_evaluations == rb.total_num_likelihood_evaluations
+
+
 def test_linreg_fp32_refinement():
     """FP32 refinement of accepted points (regression family in BF16x3): after a shell, refine_new
     overwrites every run's accepted-point logL with the plain FP32 form, which agrees with the FP64
