@@ -39,7 +39,7 @@ PEAK_FP32_MFMA = 157.3e12
 
 # HBM bytes per k_sample launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the same
 # instrumented batch (counters scaled by a 256 MiB copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
-# profiles/k_sample_mvn_d32_n2048_r1024.md and profiles/k_sample_linreg_4096x16_n1024_r256.md.
+# profiles/k_sample_mvn_d32_n2048_r512.md and profiles/k_sample_linreg_4096x16_n1024_r256.md.
 # Keyed by (workload, runs per launch); other shapes report null.
 MEASURED_TRAFFIC = {("c2", 512): 184.93e6, ("c2", 1024): 369.82e6, ("c4", 256): 32.55e6}
 PEAK_BF16_MFMA = 2.5e15
