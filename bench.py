@@ -39,9 +39,9 @@ PEAK_FP32_MFMA = 157.3e12
 
 # HBM bytes per k_sample launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the same
 # instrumented batch (counters scaled by a 256 MiB copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
-# profiles/k_sample_mvn_d32_n2048_r512.md and profiles/k_sample_linreg_4096x16_n1024_r256.md.
+# profiles/k_sample_mvn_d32_n2048_r512.md and profiles/k_sample_linreg_4096x16_n1024_r128.md.
 # Keyed by (workload, runs per launch); other shapes report null.
-MEASURED_TRAFFIC = {("c2", 512): 184.93e6, ("c2", 1024): 369.82e6, ("c4", 256): 32.55e6}
+MEASURED_TRAFFIC = {("c2", 512): 184.95e6, ("c2", 1024): 369.82e6, ("c4", 128): 17.68e6, ("c4", 256): 32.55e6}
 PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
 
@@ -492,7 +492,7 @@ def main():
     ap.add_argument("--graph-iters", type=int, default=8)
     ap.add_argument("--groups", type=int, default=0,
                     help="chosen form: the runs of a step in this many concurrent groups, each on its own "
-                         "stream (default c2 and c3: 2, others 1)")
+                         "stream (default c2, c3 and c4: 2, others 1)")
     ap.add_argument("--lpc", type=int, default=4)
     ap.add_argument("--wpb", type=int, default=4)
     ap.add_argument("--lr-terms", type=int, default=3, choices=[3, 4])
@@ -533,8 +533,9 @@ def main():
     # c3: 1024 (256: 54.7 runs/s, 512: 65.9, 1024: 74.9; the lock-step shells' tail of the longest runs
     # amortises over more runs)
     R = args.runs or {"c2": 1024, "c3": 1024, "c4": 256, "c5": 8}.get(args.config, 128)
-    # two groups on their own streams: c2 975 against 953 runs/s, c3 81.0 against 74.9 (four: 81.5)
-    G = max(1, args.groups if args.groups else {"c2": 2, "c3": 2}.get(args.config, 1))
+    # two groups on their own streams: c2 975 against 953 runs/s, c3 81.0 against 74.9 (four: 81.5),
+    # c4 68.8 against 64.0
+    G = max(1, args.groups if args.groups else {"c2": 2, "c3": 2, "c4": 2}.get(args.config, 1))
     RG = (R + G - 1) // G  # runs per group: the batch one launch of the dominant kernel covers
     seed_base = 0
     if args.config == "c5":
