@@ -449,6 +449,37 @@ def test_concurrent_batches_match_sequential():
             assert ra.total_num_likelihood_evaluations == rb.total_num_likelihood_evaluations
 
 
+def test_concurrent_batches_of_different_families():
+    """run_concurrent with a multivariate-normal batch (D=8) and a mixture batch (D=5, uniform prior)
+    on their two streams, each with its own termination condition: both bit-identical to lone runs."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec, run_concurrent
+    say("concurrent batches of two likelihood families")
+    mvn, _, _, _ = mvn_problem(8, seed=6)
+    gmm, _, _, _ = gmm_problem(5, seed=7)
+
+    def make():
+        a = DeviceBatch([RunSpec(model=mvn, num_live_points=96, num_slices=16, k_phantom=1, max_samples=96 * 120,
+                                 seed=s) for s in (11, 12)], lanes_per_chain=4, waves_per_block=2)
+        b = DeviceBatch([RunSpec(model=gmm, num_live_points=160, num_slices=10, k_phantom=0,
+                                 max_samples=160 * 150, seed=s) for s in (13, 14, 15)], lanes_per_chain=2,
+                        waves_per_block=2)
+        return [a, b]
+
+    terms = [dict(dlogZ=math.log(1.0 + 1e-3), max_samples=1e18), dict(dlogZ=math.log(1.0 + 1e-2), max_samples=1e18)]
+    con = make()
+    run_concurrent(con, terms, graph_iters=4)
+    seq = make()
+    for b, t in zip(seq, terms):
+        b.run(t, graph_iters=4)
+    for a, b in zip(con, seq):
+        assert np.array_equal(a.log_Z(), b.log_Z())
+        for r in range(a.R):
+            ra, rb = a.to_results(r), b.to_results(r)
+            assert ra.total_num_samples == rb.total_num_samples
+            assert np.array_equal(ra.log_L_samples, rb.log_L_samples)
+            assert ra.total_num_likelihood_evaluations == rb.total_num_likelihood_evaluations
+
+
 def test_linreg_fp32_refinement():
     """FP32 refinement of accepted points (regression family in BF16x3): after a shell, refine_new
     overwrites every run's accepted-point logL with the plain FP32 form, which agrees with the FP64
