@@ -342,6 +342,19 @@ int hipns_ndtri_f32(const float* u_dev, int64_t n, float* z_dev, void* stream);
  * (uni_slice_sampler.py:41-64 divides (0 - x) and (1 - x) by the direction). */
 int hipns_slice_quotients(const float* a_dev, const float* n_dev, int64_t count, float* q_dev, void* stream);
 
+/* The seed point chain i of a shell picks in iteration `iteration` of a run keyed (key0, key1), i in
+ * [0, C): idx_dev[i] and the uniform u_dev[i] it came from, for a sorted live set of N points whose
+ * entries [first, N) lie above the contour (get_seed_point, samplers/uni_slice_sampler.py:343-358, via
+ * resample_indicies, internals/random.py:34-84). */
+int hipns_seed_indices(uint32_t key0, uint32_t key1, uint32_t iteration, int32_t first, int32_t N, int32_t C,
+                       int32_t* idx_dev, double* u_dev, void* stream);
+
+/* The unit direction of every slice of chains 0..C-1 in iteration `iteration` (_sample_direction,
+ * samplers/uni_slice_sampler.py:23-38), computed in the fused kernel's lane layout with
+ * lanes_per_chain lanes per chain: dirs_dev [C][num_slices][D]. */
+int hipns_slice_directions(uint32_t key0, uint32_t key1, uint32_t iteration, int32_t C, int32_t num_slices, int32_t D,
+                           int32_t lanes_per_chain, float* dirs_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

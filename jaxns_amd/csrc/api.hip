@@ -628,4 +628,29 @@ int hipns_slice_quotients(const float* a_dev, const float* n_dev, int64_t count,
   return HIPNS_OK;
 }
 
+int hipns_seed_indices(uint32_t key0, uint32_t key1, uint32_t iteration, int32_t first, int32_t N, int32_t C,
+                       int32_t* idx_dev, double* u_dev, void* stream) {
+  if (N < 1 || first < 0 || first > N) return fail(HIPNS_EINVAL, "need N >= 1 and 0 <= first <= N");
+  if (C < 0) return fail(HIPNS_EINVAL, "C < 0");
+  if (C == 0) return HIPNS_OK;
+  if (!idx_dev || !u_dev) return fail(HIPNS_EINVAL, "null pointer");
+  hipError_t e = hipns::launch_seed_indices(key0, key1, iteration, first, N, C, idx_dev, u_dev, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "k_seed_indices");
+  return HIPNS_OK;
+}
+
+int hipns_slice_directions(uint32_t key0, uint32_t key1, uint32_t iteration, int32_t C, int32_t num_slices, int32_t D,
+                           int32_t lanes_per_chain, float* dirs_dev, void* stream) {
+  if (D < 1 || D > HIPNS_MAX_DIM) return fail(HIPNS_EINVAL, "D outside [1, %d]", HIPNS_MAX_DIM);
+  if (num_slices < 1 || num_slices > 65535) return fail(HIPNS_EINVAL, "num_slices outside [1, 65535]");
+  if (lanes_per_chain != 2 && lanes_per_chain != 4) return fail(HIPNS_EINVAL, "lanes_per_chain must be 2 or 4");
+  if (C < 0) return fail(HIPNS_EINVAL, "C < 0");
+  if (C == 0) return HIPNS_OK;
+  if (!dirs_dev) return fail(HIPNS_EINVAL, "null pointer");
+  hipError_t e = hipns::launch_slice_dirs(key0, key1, iteration, C, num_slices, D, lanes_per_chain, dirs_dev,
+                                          as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "k_slice_dirs");
+  return HIPNS_OK;
+}
+
 }  // extern "C"

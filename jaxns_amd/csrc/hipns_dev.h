@@ -488,6 +488,16 @@ __device__ __forceinline__ int determine_termination(const hipns_term_cond_t& c,
   return done;
 }
 
+// First index of `count` entries appended to a sample collection of `size` entries at `next`
+// (_add_samples_to_state, sharded_static.py:40-85, through replace_index, internals/maps.py:15-25):
+// lax.dynamic_update_slice clamps its start so that the update stays in bounds, so an append that
+// would run past the end is shifted back to end at `size` (it overwrites the entries before it) rather
+// than wrapped.  `next` itself wraps modulo `size` afterwards.
+__host__ __device__ __forceinline__ int64_t append_start(int64_t next, int64_t count, int64_t size) {
+  const int64_t last = size - count;
+  return next < last ? next : (last > 0 ? last : 0);
+}
+
 // key order of a stable FP64 sort: (key, index) lexicographic with NaN last
 __device__ __forceinline__ bool key_lt(double ka, int ia, double kb, int ib) {
   bool na = __builtin_isnan(ka), nb = __builtin_isnan(kb);

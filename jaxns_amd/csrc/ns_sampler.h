@@ -116,4 +116,8 @@ hipError_t launch_sampler_init(const SampleArgs& A, int R, int max_chains, const
                                size_t lds_bytes, hipStream_t s);
 hipError_t launch_sampler_inj(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
                               size_t lds_bytes, hipStream_t s);
+hipError_t launch_seed_indices(uint32_t k0, uint32_t k1, uint32_t it32, int first, int N, int C, int32_t* idx,
+                               double* u, hipStream_t s);
+hipError_t launch_slice_dirs(uint32_t k0, uint32_t k1, uint32_t it32, int C, int num_slices, int D, int lpc,
+                             float* out, hipStream_t s);
 }  // namespace hipns

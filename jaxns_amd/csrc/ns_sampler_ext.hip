@@ -197,8 +197,9 @@ __global__ void __launch_bounds__(256) k_ext_step(ExtArgs A) {
           // phantom samples: x[-(k+1):-1] (uni_slice_sampler.py:432-440), evals 0
           const int pidx = s - (num_slices - 1 - kph);
           if (kph > 0 && pidx >= 0 && s < num_slices - 1) {
-            const int64_t phantom_base = (st->next_idx + S) % run.max_samples;
-            const int64_t si = (phantom_base + (int64_t)chain * kph + pidx) % run.max_samples + run.samp_off;
+            const int64_t phantom_base =
+                append_start((st->next_idx + S) % run.max_samples, (int64_t)S * kph, run.max_samples);
+            const int64_t si = run.samp_off + phantom_base + (int64_t)chain * kph + pidx;
             float* pd = A.B.samp_U + si * dps;
 #pragma unroll
             for (int j = 0; j < NJ; ++j)

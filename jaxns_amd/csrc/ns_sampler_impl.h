@@ -857,7 +857,12 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   bool need_begin = (MODE == MODE_SLICE || MODE == MODE_INJ) && alive;
   bool need_shrink = false;
 
-  const int64_t phantom_base = (MODE == MODE_SLICE) ? (st->next_idx + S) % run.max_samples : 0;
+  // the shell's phantoms are appended after its discarded points (add_phantom_samples_to_state,
+  // sharded_static.py:181-207), chain-major, k per chain
+  const int64_t phantom_base =
+      (MODE == MODE_SLICE && kph > 0)
+          ? append_start((st->next_idx + S) % run.max_samples, (int64_t)S * kph, run.max_samples)
+          : 0;
   const int sender = (MODE == MODE_SLICE) ? st->sender_node : 0;
   const int max_uniform = (MODE == MODE_INIT) ? 1000000 : run.max_uniform_evals;
   u32x4 pickblk = {0u, 0u, 0u, 0u};
@@ -967,7 +972,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
                 if (j < nj) pd[owned_dim<LPC>(h, j)] = U0[j];
               if (h == 0) A.ph_logL[(int64_t)chain * kph + pidx] = L;
             } else {
-              int64_t si = (phantom_base + (int64_t)chain * kph + pidx) % run.max_samples + run.samp_off;
+              const int64_t si = run.samp_off + phantom_base + (int64_t)chain * kph + pidx;
               float* pd = A.B.samp_U + si * dps;
 #pragma unroll
               for (int j = 0; j < NJ; ++j)

@@ -1,49 +1,10 @@
-// ns_sampler_unif.hip -- instantiations of the sampling kernel for MODE_UNIF.
-#include "ns_sampler_impl.h"
+// ns_sampler_unif.hip -- the sampling kernel's MODE_UNIF instances: UniformSampler chains of a shell.
+#include "ns_sampler_dispatch.h"
 
 namespace hipns {
 
-// ---------------------------------------------------------------------------
-// host-side dispatch over the template grid
-// ---------------------------------------------------------------------------
-template <int LPC, int NJ, int FAM, int MODE>
-static hipError_t launch_one(const SampleArgs& A, int R, int max_chains, int wpb, size_t lds_bytes, hipStream_t s) {
-  constexpr int CW = HIPNS_WAVE / LPC;
-  int groups = (max_chains + CW - 1) / CW;
-  int bx = (groups + wpb - 1) / wpb;
-  if (bx < 1) bx = 1;
-  dim3 grid(bx, R), block(wpb * HIPNS_WAVE);
-  hipLaunchKernelGGL((k_sample<LPC, NJ, FAM, MODE>), grid, block, lds_bytes, s, A);
-  return hipGetLastError();
-}
-
-// register slots per lane: the smallest instance whose NJ * LPC covers the widest run (D rounded up
-// to 8 * LPC), which keeps VGPRs (and so occupancy) in step with the model dimension
-template <int FAM, int MODE>
-static hipError_t launch_lpc(const SampleArgs& A, int R, int max_chains, int lpc, int wpb, int dmax, size_t lds,
-                             hipStream_t s) {
-  if (lpc == 4 || dmax > 64) {
-    if (dmax <= 32) return launch_one<4, 8, FAM, MODE>(A, R, max_chains, wpb, lds, s);
-    if (dmax <= 64) return launch_one<4, 16, FAM, MODE>(A, R, max_chains, wpb, lds, s);
-    return launch_one<4, 32, FAM, MODE>(A, R, max_chains, wpb, lds, s);
-  }
-  if (dmax <= 16) return launch_one<2, 8, FAM, MODE>(A, R, max_chains, wpb, lds, s);
-  if (dmax <= 32) return launch_one<2, 16, FAM, MODE>(A, R, max_chains, wpb, lds, s);
-  return launch_one<2, 32, FAM, MODE>(A, R, max_chains, wpb, lds, s);
-}
-
-template <int MODE>
-static hipError_t launch_fam(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
-                             size_t lds, hipStream_t s) {
-  if (L.family == HIPNS_FAM_GMM)
-    return launch_lpc<HIPNS_FAM_GMM, MODE>(A, R, max_chains, L.lanes_per_chain, L.waves_per_block, dmax, lds, s);
-  if (L.family == HIPNS_FAM_LINREG)
-    return launch_lpc<HIPNS_FAM_LINREG, MODE>(A, R, max_chains, L.lanes_per_chain, L.waves_per_block, dmax, lds, s);
-  return launch_lpc<HIPNS_FAM_MVN, MODE>(A, R, max_chains, L.lanes_per_chain, L.waves_per_block, dmax, lds, s);
-}
-
 hipError_t launch_sampler_unif(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
-                          size_t lds_bytes, hipStream_t s) {
+                                size_t lds_bytes, hipStream_t s) {
   return launch_fam<MODE_UNIF>(A, R, max_chains, L, dmax, lds_bytes, s);
 }
 

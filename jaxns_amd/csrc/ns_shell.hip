@@ -23,6 +23,7 @@ __global__ void __launch_bounds__(256) k_discard(const hipns_run_desc_t* runs, h
   const hipns_run_desc_t run = runs[r];
   const int S = run.shell, N = run.N, dps = B.dps, cur = st->buf;
   const int64_t next = st->next_idx;
+  const int64_t start = append_start(next, S, run.max_samples);
   // one float4 of a row per thread (dps is a multiple of 32)
   const uint32_t q = (uint32_t)dps >> 2;
   const float4* srcU = (const float4*)(B.live_U[cur] + run.live_off * dps);
@@ -30,9 +31,7 @@ __global__ void __launch_bounds__(256) k_discard(const hipns_run_desc_t* runs, h
   const uint32_t total = (uint32_t)S * q;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
     const uint32_t i = e / q, c = e - i * q;
-    int64_t j = next + i;
-    if (j >= run.max_samples) j %= run.max_samples;  // _add_samples_to_state's index wrap (rare)
-    j += run.samp_off;
+    const int64_t j = run.samp_off + start + i;
     dstU[j * q + c] = srcU[e];
     if (c == 0) {
       B.samp_logL[j] = B.live_logL[cur][run.live_off + i];
@@ -291,7 +290,9 @@ __global__ void __launch_bounds__(1024) k_register(const hipns_run_desc_t* runs,
       EvMap f = evmap_step(1.0, Lp, live[N - 1], &lt);
       s2.dZ = lt + (s2.X - f.a);
     }
-    const int kph = run.k_phantom;
+    // the uniform sampler keeps no phantom samples (UniformSampler.num_phantom() = 0,
+    // samplers/uniform_samplers.py), so its shells advance the collection by S only
+    const int kph = run.sampler == 1 ? 0 : run.k_phantom;
     const int64_t added = (int64_t)S + (int64_t)S * kph;
     st->num_samples += added;
     st->next_idx = (st->next_idx + added) % run.max_samples;
@@ -339,7 +340,9 @@ __global__ void k_check_term(const hipns_run_desc_t* runs, hipns_run_state_t* st
   }
   hipns_term_cond_t c = term;
   if (c.has_max_samples) {
-    double cap = (double)run.max_samples - (double)run.shell * (1.0 + run.k_phantom);
+    // space_needed_per_iteration = shell * (1 + sampler.num_phantom()) (sharded_static.py:461-470)
+    const int kph = run.sampler == 1 ? 0 : run.k_phantom;
+    double cap = (double)run.max_samples - (double)run.shell * (1.0 + kph);
     c.max_samples = fmin(c.max_samples, cap);
   }
   int64_t why;
@@ -406,16 +409,14 @@ __global__ void __launch_bounds__(256) k_consume(const hipns_run_desc_t* runs, h
   hipns_run_state_t* st = states + r;
   const hipns_run_desc_t run = runs[r];
   const int N = run.N, dps = B.dps, cur = st->buf;
-  const int64_t next = st->next_idx;
+  const int64_t start = append_start(st->next_idx, N, run.max_samples);
   const uint32_t q = (uint32_t)dps >> 2;
   const float4* srcU = (const float4*)(B.live_U[cur] + run.live_off * dps);
   float4* dstU = (float4*)B.samp_U;
   const uint32_t total = (uint32_t)N * q;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
     const uint32_t i = e / q, c = e - i * q;
-    int64_t j = next + i;
-    if (j >= run.max_samples) j %= run.max_samples;
-    j += run.samp_off;
+    const int64_t j = run.samp_off + start + i;
     dstU[j * q + c] = srcU[e];
     if (c == 0) {
       B.samp_logL[j] = B.live_logL[cur][run.live_off + i];

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -224,6 +225,10 @@ class DeviceBatch:
                 raise ValueError("refine_fp32 applies to the regression family in a BF16 form (lr_terms 3 or 4)")
             self.launch_fp32 = Launch.from_buffer_copy(self.launch)
             self.launch_fp32.lr_terms = 1
+            # per run: refined accepted points whose FP32 logL is not above the shell's contour (the
+            # BF16 value that decided acceptance was); they enter the live set below the contour and
+            # are discarded again by the next shell
+            self.refine_below = torch.zeros(self.R, dtype=torch.int64, device=dev)
         if self.external:
             i32 = torch.int32
             self.ext_U0 = torch.full((sumN, dps), 0.5, dtype=f32, device=dev)
@@ -303,12 +308,17 @@ class DeviceBatch:
         s = self._s(stream)
         sz = C.sizeof(RunDesc)
         base_U, base_L, base_runs = ptr(self.new_U), ptr(self.new_logL), ptr(self.runs_dev)
+        torch = _torch()
+        contour = self.states_dev.view(torch.float64).view(self.R, -1)[:, RunState.contour.offset // 8]
+        done = self.states_dev.view(torch.int32).view(self.R, -1)[:, RunState.done.offset // 4]
         for r in range(self.R):
             d = self.runs_host[r]
             n, off = int(d.shell), int(d.chain_off)
             check(lib().hipns_log_likelihood(base_U + off * self.dps * 4, n, base_L + off * 8, base_runs + r * sz,
                                              C.pointer(self.runs_host[r]), C.byref(self.bufs),
                                              C.byref(self.launch_fp32), s), "hipns_log_likelihood")
+            below = (self.new_logL[off:off + n] <= contour[r]).sum() * (done[r] == 0)
+            self.refine_below[r:r + 1] += below
 
     def collect_shell(self, term: dict, stream=None):
         tc = _lib.make_term_cond(**term)
@@ -349,6 +359,11 @@ class DeviceBatch:
         off = RunState.done.offset
         flags = st[:, off:off + 4].contiguous().view(torch.int32).cpu().numpy().ravel()
         return flags
+
+    def cap_hits(self) -> np.ndarray:
+        """Per run: slices that reached the step cap (max_steps shrinks) and kept their seed point.
+        The reference's shrink loop has no cap, so a non-zero count means the run left its algorithm."""
+        return np.array([int(st.cap_hits) for st in self.states()], dtype=np.int64)
 
     def set_sampler(self, sampler: int):
         for r in range(self.R):
@@ -505,6 +520,8 @@ class DeviceBatch:
         out = dict(log_Z_mean=float(log_Z_mean), log_Z_uncert=float(log_Z_uncert), ESS=float(ess), M=M,
                    total_phantom=total_phantom, termination_reason=int(st.termination_reason),
                    iterations=int(st.iteration), cap_hits=int(st.cap_hits))
+        if self.refine_fp32:
+            out["refined_below_contour"] = int(self.refine_below[r].item())
         if full:
             ev = self.samp_evals[off:off + M].index_select(0, si)
             U = self.samp_U[off:off + M].index_select(0, si)[:, :spec.model.U_ndims]
@@ -515,6 +532,9 @@ class DeviceBatch:
 
     def to_results(self, r: int, stream=None) -> NestedSamplerResults:
         core = self.results_core(r, stream=stream, full=True)
+        if core["cap_hits"]:
+            warnings.warn(f"run {r}: {core['cap_hits']} slices reached the step cap of {self.launch.max_steps} shrinks "
+                          f"and kept their seed point; raise max_steps", RuntimeWarning)
         model = self.specs[r].model
         per8 = core["per8"]
         log_L = core["log_L"]
@@ -559,7 +579,12 @@ def run_concurrent(batches: List["DeviceBatch"], term, *, init_efficiency_thresh
     if graph_iters <= 0:
         raise ValueError("run_concurrent needs graph_iters > 0")
     streams = [torch.cuda.Stream() for _ in batches]
-    uterms = [dict(efficiency_threshold=float(init_efficiency_threshold), dlogZ=0.0,
+    # the side streams do not wait for the default stream by themselves: DeviceBatch.__init__ fills
+    # the live set, new chains and run states there, and every batch's first kernel must follow that
+    cur = torch.cuda.current_stream()
+    for s in streams:
+        s.wait_stream(cur)
+    uterms =[dict(efficiency_threshold=float(init_efficiency_threshold), dlogZ=0.0,
                    max_samples=float(min(sp.max_samples for sp in b.specs))) for b in batches]
     for b, s, ut in zip(batches, streams, uterms):
         with torch.cuda.stream(s):

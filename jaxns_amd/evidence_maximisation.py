@@ -219,7 +219,7 @@ class EvidenceMaximisation:
         log_Z = float("nan")
         for epoch in range(self.max_num_epochs):
             loss = opt.step(closure)
-            log_Z = -float(loss)
+            log_Z = -float(loss.detach())
             cur = theta.detach().cpu().numpy().copy()
             l_oo = float(np.max(np.abs(cur - last)))
             last = cur

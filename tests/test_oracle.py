@@ -287,3 +287,93 @@ def test_linreg_loglik_vs_scipy():
     W = 2.0 * ndtri(U)
     want = np.array([stats.norm.logpdf(lik.y, lik.X @ w, 0.5).sum() for w in W])
     np.testing.assert_allclose(got, want, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- ESS, prior density
+def test_ess_kish():
+    """effective_sample_size_kish (stats.py:77-86) = (sum dZ)^2 / sum dZ^2: M for equal weights, and the
+    ratio of the direct sums for random ones."""
+    for M in (1, 7, 1000):
+        w = np.full(M, 0.37)
+        assert O.ess_kish(np.log(w.sum()), np.log((w * w).sum())) == pytest.approx(M, rel=1e-12)
+    rng = np.random.default_rng(3)
+    w = rng.exponential(size=500) ** 3
+    want = w.sum() ** 2 / (w * w).sum()
+    assert O.ess_kish(np.log(w.sum()), np.log((w * w).sum())) == pytest.approx(want, rel=1e-12)
+
+
+def test_log_prob_prior_vs_scipy():
+    """Model.log_prob_prior (framework/model.py:178-185): the prior log density at X(U), against scipy's
+    logpdf of the Normal, Uniform and lower-triangular multivariate normal priors."""
+    rng = np.random.default_rng(5)
+    D = 4
+    U = rng.uniform(0.02, 0.98, size=(20, D))
+    loc, scale = rng.normal(size=D), rng.uniform(0.5, 2.0, size=D)
+    spec = O.ModelSpec(O.FAM_MVN, D, O.PRIOR_NORMAL, loc, scale, mvn_mu=np.zeros(D), mvn_chol=np.eye(D))
+    X = loc + scale * ndtri(U)
+    np.testing.assert_allclose(spec.log_prob_prior(U), stats.norm.logpdf(X, loc, scale).sum(1), rtol=1e-12)
+    low, width = rng.normal(size=D), rng.uniform(0.5, 3.0, size=D)
+    spec = O.ModelSpec(O.FAM_MVN, D, O.PRIOR_UNIFORM, low, width, mvn_mu=np.zeros(D), mvn_chol=np.eye(D))
+    want = stats.uniform.logpdf(low + width * U, low, width).sum(1)
+    np.testing.assert_allclose(spec.log_prob_prior(U), want, rtol=1e-12)
+    A = rng.normal(size=(D, D))
+    L = np.linalg.cholesky(A @ A.T + D * np.eye(D))
+    spec = O.ModelSpec(O.FAM_MVN, D, O.PRIOR_MVN, loc, np.ones(D), prior_tril=L, mvn_mu=np.zeros(D),
+                       mvn_chol=np.eye(D))
+    X = loc + ndtri(U) @ L.T
+    want = stats.multivariate_normal(loc, L @ L.T).logpdf(X)
+    np.testing.assert_allclose(spec.log_prob_prior(U), want, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- slice rules written out by hand
+def _unit_spec():
+    return O.ModelSpec(O.FAM_MVN, 1, O.PRIOR_NORMAL, np.zeros(1), np.ones(1), mvn_mu=np.zeros(1),
+                       mvn_chol=np.eye(1))
+
+
+def test_midpoint_shrink_known_answer():
+    """One 1-D slice from U0 = 0.5 (direction 1, bracket [-0.5, 0.5]) with replayed likelihood values
+    (reject, reject, accept) and picks u = 0.9, 0.1, 0.7, every step written out from
+    _pick_point_in_interval / _shrink_interval (uni_slice_sampler.py:67-111) with alpha = 0.5 (one slice:
+    linspace(0.5, 1, 1) = [0.5]):
+      t1 = -0.5 + 0.9 * 1.0 = 0.4                rejected, t > 0: right = 0.5 * 0.4 = 0.2
+      t2 = -0.5 + 0.1 * (0.2 + 0.5) = -0.43      rejected, t < 0: left = 0.5 * -0.43 = -0.215
+      t3 = -0.215 + 0.7 * (0.2 + 0.215) = 0.0755 accepted: U = 0.5755
+    Without the midpoint rule the bounds move to t itself:
+      t2 = -0.5 + 0.1 * 0.9 = -0.41, t3 = -0.41 + 0.7 * (0.4 + 0.41) = 0.157."""
+    spec = _unit_spec()
+    tu = np.array([[0.9, 0.1, 0.7] + [0.5] * 5])
+    rep = np.array([[-np.inf, -np.inf, 1.0] + [1.0] * 5])
+    o = O.slice_chain(spec, np.array([0.5]), 0.5, 0.0, 1, np.ones((1, 1)), tu, logL_replay=rep)
+    np.testing.assert_allclose(o["trace_t"][0, :3], [0.4, -0.43, 0.0755], rtol=0, atol=1e-15)
+    assert o["steps"][0] == 3 and o["evals"] == 3
+    assert o["U"][0] == pytest.approx(0.5755, abs=1e-15) and o["logL"] == 1.0
+    o = O.slice_chain(spec, np.array([0.5]), 0.5, 0.0, 1, np.ones((1, 1)), tu, logL_replay=rep,
+                      midpoint_shrink=False)
+    np.testing.assert_allclose(o["trace_t"][0, :3], [0.4, -0.41, 0.157], rtol=0, atol=1e-15)
+    # two slices: alpha = linspace(0.5, 1, 2) = [0.5, 1.0]; the second slice starts from the first's
+    # point 0.5755 (bracket [-0.5755, 0.4245]) and with alpha = 1 shrinks to the rejected point
+    tu2 = np.array([[0.9, 0.1, 0.7] + [0.5] * 5, [0.0, 0.5] + [0.5] * 6])
+    rep2 = np.array([[-np.inf, -np.inf, 1.0] + [1.0] * 5, [-np.inf, 2.0] + [2.0] * 6])
+    o = O.slice_chain(spec, np.array([0.5]), 0.5, 0.0, 2, np.ones((2, 1)), tu2, logL_replay=rep2)
+    np.testing.assert_allclose(o["trace_t"][1, :2], [-0.5755, -0.5755 + 0.5 * (0.4245 + 0.5755)], atol=1e-15)
+    assert o["logL"] == 2.0 and o["evals"] == 5
+
+
+def test_plateau_acceptance_rule():
+    """_new_proposal's loop condition (uni_slice_sampler.py:160-166): a proposal is accepted when
+    logL > contour, or when the seed and the proposal both sit exactly on the contour (a plateau);
+    a proposal on the contour from a seed above it is rejected."""
+    spec = _unit_spec()
+    tu = np.full((1, 8), 0.25)
+    # seed on the contour, proposal on the contour: accepted at once
+    o = O.slice_chain(spec, np.array([0.5]), 1.0, 1.0, 1, np.ones((1, 1)), tu, logL_replay=np.full((1, 8), 1.0))
+    assert o["steps"][0] == 1 and o["logL"] == 1.0
+    # seed above the contour: proposals on the contour are rejected until one is above it
+    rep = np.array([[1.0, 1.0, 1.5, 0, 0, 0, 0, 0]])
+    o = O.slice_chain(spec, np.array([0.5]), 2.0, 1.0, 1, np.ones((1, 1)), tu, logL_replay=rep)
+    assert o["steps"][0] == 3 and o["logL"] == 1.5
+    # seed on the contour, proposal below it: rejected
+    rep = np.array([[0.5, 1.0, 0, 0, 0, 0, 0, 0]])
+    o = O.slice_chain(spec, np.array([0.5]), 1.0, 1.0, 1, np.ones((1, 1)), tu, logL_replay=rep)
+    assert o["steps"][0] == 2 and o["logL"] == 1.0
