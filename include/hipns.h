@@ -237,6 +237,27 @@ int hipns_batch_evidence(const hipns_run_desc_t* runs_dev, const hipns_run_state
                          int64_t* phantom_dev, void* workspace, size_t workspace_bytes, void* stream);
 size_t hipns_batch_evidence_workspace_size(int32_t R, const hipns_run_desc_t* runs_host);
 
+/* The per-sample finals of _to_results (sharded_static.py:706-745) over M samples in tree order, given
+ * the evidence scan's per-sample log_dZ_mean and log_X_mean fields: log_dp_dev[i] = log dZ_i normalised
+ * over all samples (normalise_log_space), and H_dev[4] = {H unstable = log Z - sum dp log L,
+ * H stable = sum dp log X, H_mean (the unstable form where finite, else the stable one), log of the
+ * normaliser}.  Deterministic reductions in a fixed grouping. */
+int hipns_results_finals(const double* log_dZ_dev, const double* log_L_dev, const double* log_X_dev, int64_t M,
+                         double log_Z_mean, double* log_dp_dev, double* H_dev, void* workspace, size_t workspace_bytes,
+                         void* stream);
+size_t hipns_results_finals_workspace_size(int64_t M);
+
+/* Model.transform and log_posterior_density of _to_results (sharded_static.py:721-745, framework/model.py:
+ * 155-185) for M samples: row b reads U row perm_dev[b] (or b when perm_dev is NULL) of samp_U_dev (FP32,
+ * stride ldu); rows at or past `valid` are the collection's placeholders, U = 0.5.  With q_i = ndtri(U_i)
+ * where qkind_dev[i] != 0, else U_i: U_dev [M][D] = U (FP64), X_dev [M][D] = loc + S q (S_dev [D][D]
+ * lower triangular with a positive diagonal, row-major; diag != 0 reads its diagonal only), and
+ * log_post_dev[b] = log_L_dev[b] + log prior density (sum of -q^2/2 - log(2 pi)/2 over the ndtri slots,
+ * minus the sum of log S_ii).  Any of U_dev, X_dev, log_post_dev may be NULL. */
+int hipns_prior_finals(const float* samp_U_dev, int32_t ldu, const int64_t* perm_dev, int64_t valid, int64_t M, int32_t D,
+                       const double* loc_dev, const double* S_dev, const int32_t* qkind_dev, int32_t diag,
+                       const double* log_L_dev, double* U_dev, double* X_dev, double* log_post_dev, void* stream);
+
 /* Stable argsort of FP64 keys (jnp.argsort, stable=True; -inf < ... < +inf < NaN). */
 int hipns_argsort_stable(const double* keys_dev, int64_t n, int64_t* perm_dev, void* workspace,
                          size_t workspace_bytes, void* stream);

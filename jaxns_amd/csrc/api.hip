@@ -628,6 +628,34 @@ int hipns_slice_quotients(const float* a_dev, const float* n_dev, int64_t count,
   return HIPNS_OK;
 }
 
+int hipns_results_finals(const double* log_dZ_dev, const double* log_L_dev, const double* log_X_dev, int64_t M,
+                         double log_Z_mean, double* log_dp_dev, double* H_dev, void* workspace, size_t workspace_bytes,
+                         void* stream) {
+  if (M < 1) return fail(HIPNS_EINVAL, "M < 1");
+  if (!log_dZ_dev || !log_L_dev || !log_X_dev || !log_dp_dev || !H_dev) return fail(HIPNS_EINVAL, "null pointer");
+  if (!workspace || workspace_bytes < hipns::results_finals_ws(M))
+    return fail(HIPNS_ENOMEM, "workspace of %zu bytes needed", hipns::results_finals_ws(M));
+  hipError_t e = hipns::results_finals(log_dZ_dev, log_L_dev, log_X_dev, M, log_Z_mean, log_dp_dev, H_dev, workspace,
+                                       as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "results finals");
+  return HIPNS_OK;
+}
+
+size_t hipns_results_finals_workspace_size(int64_t M) { return M < 1 ? 0 : hipns::results_finals_ws(M); }
+
+int hipns_prior_finals(const float* samp_U_dev, int32_t ldu, const int64_t* perm_dev, int64_t valid, int64_t M, int32_t D,
+                       const double* loc_dev, const double* S_dev, const int32_t* qkind_dev, int32_t diag,
+                       const double* log_L_dev, double* U_dev, double* X_dev, double* log_post_dev, void* stream) {
+  if (M < 0 || D < 1 || ldu < D) return fail(HIPNS_EINVAL, "need M >= 0, D >= 1, ldu >= D");
+  if (M == 0) return HIPNS_OK;
+  if (!samp_U_dev || !loc_dev || !S_dev || !qkind_dev) return fail(HIPNS_EINVAL, "null pointer");
+  if (log_post_dev && !log_L_dev) return fail(HIPNS_EINVAL, "log_post needs log_L");
+  hipError_t e = hipns::prior_finals(samp_U_dev, ldu, perm_dev, valid, M, D, loc_dev, S_dev, qkind_dev, diag, log_L_dev,
+                                     U_dev, X_dev, log_post_dev, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "k_prior_finals");
+  return HIPNS_OK;
+}
+
 int hipns_seed_indices(uint32_t key0, uint32_t key1, uint32_t iteration, int32_t first, int32_t N, int32_t C,
                        int32_t* idx_dev, double* u_dev, void* stream) {
   if (N < 1 || first < 0 || first > N) return fail(HIPNS_EINVAL, "need N >= 1 and 0 <= first <= N");

@@ -66,6 +66,8 @@ static hipError_t launch_dirs_one(uint32_t k0, uint32_t k1, uint32_t it32, int C
 
 hipError_t launch_slice_dirs(uint32_t k0, uint32_t k1, uint32_t it32, int C, int num_slices, int D, int lpc,
                              float* out, hipStream_t s) {
+  // the fused kernel's instance for this dimension (launch_lpc: beyond 64 dimensions, 4 lanes per chain)
+  if (D > 64) lpc = 4;
   const int nj = sampler_nj(D, lpc);
   if (lpc == 4) {
     if (nj == 8) return launch_dirs_one<4, 8>(k0, k1, it32, C, num_slices, D, out, s);

@@ -33,4 +33,12 @@ size_t batch_evidence_ws(int R, int64_t max_M);
 hipError_t batch_evidence(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int R, int64_t max_M,
                           const hipns_buffers_t& B, double* fin, int64_t* phantom, void* ws, hipStream_t s);
 
+// _to_results finals (ns_finals.hip)
+size_t results_finals_ws(int64_t M);
+hipError_t results_finals(const double* log_dZ, const double* log_L, const double* log_X, int64_t M,
+                          double log_Z_mean, double* log_dp, double* H_out, void* ws, hipStream_t s);
+hipError_t prior_finals(const float* samp_U, int ldu, const int64_t* perm, int64_t valid, int64_t M, int D,
+                        const double* loc, const double* S, const int32_t* qkind, int diag, const double* log_L,
+                        double* U_out, double* X_out, double* log_post, hipStream_t s);
+
 }  // namespace hipns

@@ -83,8 +83,10 @@ __global__ void __launch_bounds__(1024) k_replace(const hipns_run_desc_t* runs, 
   int* idx = (int*)(keys + P);
   int* perm = idx + P;
   const int tid = threadIdx.x, nt = blockDim.x;
+  // padding slots hold NaN with indices >= S: they sort after every real key, NaN included (NaN keys
+  // tie-break on the index)
   for (int i = tid; i < P; i += nt) {
-    keys[i] = i < S ? B.new_logL[run.chain_off + i] : __builtin_inf();
+    keys[i] = i < S ? B.new_logL[run.chain_off + i] : __builtin_nan("");
     idx[i] = i;
   }
   __syncthreads();

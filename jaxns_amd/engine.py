@@ -486,78 +486,116 @@ class DeviceBatch:
             out.append((mu, math.sqrt(var) * math.sqrt(1.0 + k)))
         return np.array(out), fin, ph
 
-    def results_core(self, r: int, stream=None, full: bool = False) -> dict:
-        """_to_results (sharded_static.py:652-745, trim=True) for run r, on the device."""
+    def results_core(self, r: int, stream=None, full: bool = False, trim: bool = True) -> dict:
+        """_to_results (sharded_static.py:652-745) for run r on the device: count_crossed_edges, the
+        evidence scan and, with full=True, the per-sample finals (hipns_results_finals,
+        hipns_prior_finals).  trim=True works on the first num_samples entries of the sample collection;
+        trim=False on the whole collection (max_samples entries) with the dynamic forms that stop at
+        num_samples (count_crossed_edges(num_samples=...), compute_evidence_stats(num_samples=...)),
+        entries past it being the collection's placeholders."""
         torch = _torch()
         st = self.states()[r]
         spec = self.specs[r]
         d = self.runs_host[r]
         M = int(min(st.num_samples, d.max_samples))
+        L = M if trim else int(d.max_samples)
+        stop = -1 if trim else M
         off = int(d.samp_off)
         dev = self.device
-        logL = self.samp_logL[off:off + M]
-        sender = self.samp_sender[off:off + M]
-        si = torch.empty(M, dtype=torch.int64, device=dev)
-        nl = torch.empty(M, dtype=torch.int64, device=dev)
-        ws = torch.empty(int(lib().hipns_count_crossed_edges_workspace_size(M)), dtype=torch.uint8, device=dev)
-        check(lib().hipns_count_crossed_edges(ptr(sender), ptr(logL), M, -1, ptr(si), ptr(nl), ptr(ws), ws.numel(),
-                                              self._s(stream)), "hipns_count_crossed_edges")
+        s = self._s(stream)
+        logL = self.samp_logL[off:off + L]
+        sender = self.samp_sender[off:off + L]
+        si = torch.empty(L, dtype=torch.int64, device=dev)
+        nl = torch.empty(L, dtype=torch.int64, device=dev)
+        ws = torch.empty(int(lib().hipns_count_crossed_edges_workspace_size(L)), dtype=torch.uint8, device=dev)
+        check(lib().hipns_count_crossed_edges(ptr(sender), ptr(logL), L, stop, ptr(si), ptr(nl), ptr(ws), ws.numel(),
+                                              s), "hipns_count_crossed_edges")
         lL = logL.index_select(0, si)
         nld = nl.to(torch.float64)
-        per8 = torch.empty((8, M), dtype=torch.float64, device=dev)
+        per8 = torch.empty((8, L), dtype=torch.float64, device=dev)
         fin8 = torch.empty(8, dtype=torch.float64, device=dev)
-        ws2 = torch.empty(int(lib().hipns_evidence_scan_workspace_size(M)), dtype=torch.uint8, device=dev)
-        check(lib().hipns_evidence_scan(ptr(lL), ptr(nld), M, -1, ptr(per8), ptr(fin8), ptr(ws2), ws2.numel(),
-                                        self._s(stream)), "hipns_evidence_scan")
+        ws2 = torch.empty(int(lib().hipns_evidence_scan_workspace_size(L)), dtype=torch.uint8, device=dev)
+        check(lib().hipns_evidence_scan(ptr(lL), ptr(nld), L, stop, ptr(per8), ptr(fin8), ptr(ws2), ws2.numel(), s),
+              "hipns_evidence_scan")
         fin = fin8.cpu().numpy()
         log_Z_mean, log_Z_var = _linear_to_log_stats(fin[3], fin[5])
-        ph = self.samp_phantom[off:off + M]
-        total_phantom = int(ph.to(torch.int64).sum().item())
+        total_phantom = int(self.samp_phantom[off:off + M].to(torch.int64).sum().item())
         pf = total_phantom / M
         k = pf / (1.0 - pf)
         log_Z_uncert = math.sqrt(log_Z_var) * math.sqrt(1.0 + k)
         ess = math.exp((fin[3] + fin[3]) - fin[7]) / (1.0 + k)
-        out = dict(log_Z_mean=float(log_Z_mean), log_Z_uncert=float(log_Z_uncert), ESS=float(ess), M=M,
+        out = dict(log_Z_mean=float(log_Z_mean), log_Z_uncert=float(log_Z_uncert), ESS=float(ess), M=M, L=L,
                    total_phantom=total_phantom, termination_reason=int(st.termination_reason),
                    iterations=int(st.iteration), cap_hits=int(st.cap_hits))
         if self.refine_fp32:
             out["refined_below_contour"] = int(self.refine_below[r].item())
         if full:
-            ev = self.samp_evals[off:off + M].index_select(0, si)
-            U = self.samp_U[off:off + M].index_select(0, si)[:, :spec.model.U_ndims]
+            model = spec.model
+            D = model.U_ndims
+            ev = self.samp_evals[off:off + L].index_select(0, si)
+            log_dp = torch.empty(L, dtype=torch.float64, device=dev)
+            H = torch.empty(4, dtype=torch.float64, device=dev)
+            ws3 = torch.empty(int(lib().hipns_results_finals_workspace_size(L)), dtype=torch.uint8, device=dev)
+            check(lib().hipns_results_finals(ptr(per8[6]), ptr(lL), ptr(per8[1]), L, float(log_Z_mean), ptr(log_dp),
+                                             ptr(H), ptr(ws3), ws3.numel(), s), "hipns_results_finals")
+            U = torch.empty((L, D), dtype=torch.float64, device=dev)
+            if model.dynamic_prior:
+                # a prior program that depends on earlier prior values: the model's generator forms X
+                check(lib().hipns_prior_finals(ptr(self.samp_U) + off * self.dps * 4, self.dps, ptr(si), M, L, D,
+                                               *self._prior_program_dev(model), None, ptr(U), None, None, s),
+                      "hipns_prior_finals")
+                Uh = U.cpu().numpy()
+                X = None
+                log_post = lL.cpu().numpy() + model.log_prob_prior_numpy(Uh)
+            else:
+                X = torch.empty((L, D), dtype=torch.float64, device=dev)
+                lpost = torch.empty(L, dtype=torch.float64, device=dev)
+                check(lib().hipns_prior_finals(ptr(self.samp_U) + off * self.dps * 4, self.dps, ptr(si), M, L, D,
+                                               *self._prior_program_dev(model), ptr(lL), ptr(U), ptr(X), ptr(lpost),
+                                               s), "hipns_prior_finals")
+                Uh = U.cpu().numpy()
+                X = X.cpu().numpy()
+                log_post = lpost.cpu().numpy()
+            Hh = H.cpu().numpy()
             out.update(samples_indices=si.cpu().numpy(), num_live_points=nl.cpu().numpy(), log_L=lL.cpu().numpy(),
-                       per8=per8.cpu().numpy(), evals=ev.to(torch.int64).cpu().numpy(),
-                       U=U.to(torch.float64).cpu().numpy())
+                       per8=per8.cpu().numpy(), evals=ev.to(torch.int64).cpu().numpy(), U=Uh, X=X,
+                       log_dp=log_dp.cpu().numpy(), H_unstable=float(Hh[0]), H_stable=float(Hh[1]),
+                       H_mean=float(Hh[2]), log_posterior_density=log_post)
         return out
 
-    def to_results(self, r: int, stream=None) -> NestedSamplerResults:
-        core = self.results_core(r, stream=stream, full=True)
+    def _prior_program_dev(self, model):
+        """(loc, S, qkind, diag) of the model's affine prior program as device arrays (cached per model)."""
+        torch = _torch()
+        cache = self.__dict__.setdefault("_prior_dev", {})
+        key = id(model)
+        if key not in cache:
+            qkind, loc, S = model.prior_program()
+            diag = int(np.allclose(S, np.diag(np.diag(S))))
+            t = (torch.as_tensor(np.ascontiguousarray(loc), dtype=torch.float64, device=self.device),
+                 torch.as_tensor(np.ascontiguousarray(S), dtype=torch.float64, device=self.device),
+                 torch.as_tensor(np.ascontiguousarray(qkind), dtype=torch.int32, device=self.device), diag)
+            cache[key] = (model, t)
+        _, (loc, S, qk, diag) = cache[key]
+        return ptr(loc), ptr(S), ptr(qk), diag
+
+    def to_results(self, r: int, stream=None, trim: bool = True) -> NestedSamplerResults:
+        """NestedSamplerResults of run r (sharded_static.py:652-773), every per-sample field formed on the
+        device; trim=False keeps the whole collection (max_samples rows, placeholders past num_samples)."""
+        core = self.results_core(r, stream=stream, full=True, trim=trim)
         if core["cap_hits"]:
             warnings.warn(f"run {r}: {core['cap_hits']} slices reached the step cap of {self.launch.max_steps} shrinks "
                           f"and kept their seed point; raise max_steps", RuntimeWarning)
         model = self.specs[r].model
-        per8 = core["per8"]
-        log_L = core["log_L"]
-        log_dZ = per8[6]
-        log_X = per8[1]
-        mx = np.max(log_dZ)
-        lnorm = mx + np.log(np.sum(np.exp(log_dZ - mx))) if np.isfinite(mx) else -np.inf
-        log_dp = np.where(np.isneginf(lnorm), -np.inf, log_dZ - lnorm)
-        p = np.exp(log_dp)
-        v = np.where(np.isneginf(log_dp), 0.0, log_L)
-        H_instable = -(np.sum(p * v) - core["log_Z_mean"])
-        H_stable = -np.sum(p * (-log_X))
-        H = H_instable if np.isfinite(H_instable) else H_stable
+        U = core["U"]
+        samples = model.transform(U) if core["X"] is None else model.split_transformed(core["X"])
         evals = core["evals"]
         total_evals = int(np.sum(evals))
         M = core["M"]
-        U = core["U"]
-        samples = model.transform(U)
-        log_post = log_L + model.log_prob_prior_numpy(U)
         return NestedSamplerResults(
-            log_Z_mean=core["log_Z_mean"], log_Z_uncert=core["log_Z_uncert"], ESS=core["ESS"], H_mean=float(H),
-            samples=samples, parametrised_samples={}, U_samples=U, log_L_samples=log_L, log_dp_mean=log_dp,
-            log_X_mean=log_X, log_posterior_density=log_post, num_live_points_per_sample=core["num_live_points"],
+            log_Z_mean=core["log_Z_mean"], log_Z_uncert=core["log_Z_uncert"], ESS=core["ESS"], H_mean=core["H_mean"],
+            samples=samples, parametrised_samples={}, U_samples=U, log_L_samples=core["log_L"],
+            log_dp_mean=core["log_dp"], log_X_mean=core["per8"][1],
+            log_posterior_density=core["log_posterior_density"], num_live_points_per_sample=core["num_live_points"],
             num_likelihood_evaluations_per_sample=evals, total_num_samples=M,
             total_phantom_samples=core["total_phantom"], total_num_likelihood_evaluations=total_evals,
             log_efficiency=float(np.log(M) - np.log(total_evals)), termination_reason=core["termination_reason"])

@@ -190,6 +190,20 @@ class Model:
             o += w
         return out
 
+    def split_transformed(self, X: np.ndarray):
+        """The dict of Model.transform from X = loc + S q(U) formed elsewhere (rows = samples): each
+        prior's columns under its name, a scalar prior's as a vector."""
+        out = {}
+        o = 0
+        for i, p in enumerate(self.priors):
+            w = p.base_ndims
+            x = X[:, o:o + w]
+            if getattr(p.dist, "event_shape", None) == ():
+                x = x[:, 0]
+            out[p.name if p.name is not None else f"_{i}"] = x
+            o += w
+        return out
+
     def forward_numpy(self, U: np.ndarray) -> np.ndarray:
         """Model.forward on the host in float64 (framework/model.py:167-176); NaN -> -inf."""
         U = np.atleast_2d(np.asarray(U, dtype=np.float64))

@@ -134,13 +134,22 @@ class NestedSampler:
 
     def to_results(self, termination_reason: int, state: NestedSamplerState, trim: bool = True
                    ) -> NestedSamplerResults:
-        if not trim:
-            raise NotImplementedError("trim=False is not supported; results are always trimmed")
-        return state.batch.to_results(state.run_index)
+        """public.py:177-196: trim=False keeps every row of the sample collection (max_samples), the
+        statistics stopping at total_num_samples; trim_results cuts such results to that length."""
+        return state.batch.to_results(state.run_index, trim=trim)
 
     @staticmethod
     def trim_results(results: NestedSamplerResults) -> NestedSamplerResults:
-        return results
+        """public.py:198-219: every per-sample array (size > 1) cut to total_num_samples."""
+        n = int(results.total_num_samples)
+
+        def trim(x):
+            if isinstance(x, dict):
+                return {k: trim(v) for k, v in x.items()}
+            x = np.asarray(x)
+            return x[:n] if x.size > 1 else x
+
+        return NestedSamplerResults(**{k: trim(v) for k, v in results._asdict().items()})
 
 
 DefaultNestedSampler = NestedSampler

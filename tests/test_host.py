@@ -1,6 +1,7 @@
 """Host-side logic (no GPU): C-ABI library, model parsing and parameter packing,
 public defaults, termination conditions and the results file format."""
 import ctypes as C
+import io
 import os
 import re
 
@@ -199,18 +200,33 @@ def test_runspec_rounding():
     assert s.shell == 150 and s.max_samples % (150 * 3) == 0
 
 
-def test_results_json_roundtrip(tmp_path):
-    res = J.NestedSamplerResults(
+def _small_results():
+    return J.NestedSamplerResults(
         log_Z_mean=-1.5, log_Z_uncert=0.1, ESS=100.0, H_mean=-2.0, samples={"x": np.arange(6.0).reshape(3, 2)},
         parametrised_samples={}, U_samples=np.random.default_rng(0).uniform(size=(3, 2)),
         log_L_samples=np.array([-np.inf, 0.0, 1.0]), log_dp_mean=np.array([-np.inf, -1.0, -0.5]),
-        log_X_mean=np.array([-0.1, -0.2, -0.3]), log_posterior_density=np.zeros(3),
+        log_X_mean=np.array([-0.1, -0.2, -0.3]), log_posterior_density=np.array([-np.inf, 0.5, 0.25]),
         num_live_points_per_sample=np.array([3, 2, 1]), num_likelihood_evaluations_per_sample=np.array([1, 5, 7]),
         total_num_samples=3, total_phantom_samples=0, total_num_likelihood_evaluations=13, log_efficiency=-1.0,
         termination_reason=4)
+
+
+def test_results_json_roundtrip(tmp_path):
+    """save_results / load_results (utils.py:602-641) write jaxns' namedtuple schema
+    (namedtuple_utils.py:34-103): the class path of jaxns' NestedSamplerResults and base64 arrays with
+    dtype and shape; every field comes back equal, and its dtype and shape survive."""
+    import json
+    res = _small_results()
     f = str(tmp_path / "r.json")
     J.save_results(res, f)
+    doc = json.load(open(f))
+    assert doc["type"] == "__namedtuple__"
+    assert doc["__class__"] == "jaxns.nested_samplers.common.types.NestedSamplerResults"
+    assert set(doc["__data__"]) == set(res._fields)
+    a = doc["__data__"]["U_samples"]
+    assert a["type"] == "__ndarray__" and a["__dtype__"] == "float64" and a["__shape__"] == [3, 2]
     back = J.load_results(f)
+    assert isinstance(back, J.NestedSamplerResults)
     for k in res._fields:
         a, b = getattr(res, k), getattr(back, k)
         if isinstance(a, dict):
@@ -218,8 +234,75 @@ def test_results_json_roundtrip(tmp_path):
                 np.testing.assert_array_equal(a[kk], b[kk])
         else:
             np.testing.assert_array_equal(a, b)
+            assert np.asarray(a).dtype == np.asarray(b).dtype or np.ndim(a) == 0
     out = J.summary(back)
-    assert "logZ=-1.50 +- 0.10" in out
+    assert "logZ=-1.5 +- 0.1" in out and "max(logL)=1.0" in out
+
+
+def test_loads_a_results_file_in_the_jaxns_schema():
+    """A results file laid out as jaxns writes it (jax arrays as __jax_ndarray__, 0-d scalars; fixture
+    tests/golden/jaxns_results_schema.json) loads into this package's NestedSamplerResults with the
+    fixture's values; written back, it keeps the class path, the fields and every array."""
+    import json
+    import tempfile
+    path = os.path.join(ROOT, "tests", "golden", "jaxns_results_schema.json")
+    res = J.load_results(path)
+    assert isinstance(res, J.NestedSamplerResults)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mkfix", os.path.join(ROOT, "tests", "golden",
+                                                                         "make_results_fixture.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    want = mk.values()
+    for k, v in want.items():
+        got = getattr(res, k)
+        if isinstance(v, dict):
+            assert set(got) == set(v)
+            for kk in v:
+                np.testing.assert_array_equal(got[kk], v[kk])
+        else:
+            np.testing.assert_array_equal(got, v)
+            assert np.shape(got) == np.shape(v)
+    assert int(res.total_num_samples) == 6 and float(res.log_Z_mean) == -3.25
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "again.json")
+        J.save_results(res, f)
+        a, b = json.load(open(path)), json.load(open(f))
+        assert b["__class__"] == a["__class__"] and set(b["__data__"]) == set(a["__data__"])
+        for k, v in a["__data__"].items():
+            if "type" in v:
+                assert {kk: v[kk] for kk in ("__dtype__", "__data__", "__shape__")} == \
+                    {kk: b["__data__"][k][kk] for kk in ("__dtype__", "__data__", "__shape__")}, k
+    J.summary(res)
+
+
+def test_unknown_namedtuple_loads_without_import(tmp_path):
+    """deserialise_namedtuple never imports the stored class path: an unknown NamedTuple comes back
+    as a plain namedtuple with the stored fields (the reference's test_gh171 pytree)."""
+    import json
+    from jaxns_amd.utils import load_pytree
+    f = str(tmp_path / "p.json")
+    json.dump({"type": "__namedtuple__", "__class__": "some.module.that.does.not.exist.MockPyTree171",
+               "__data__": {"x": {"type": "__jax_ndarray__", "__dtype__": "float32",
+                                  "__data__": __import__("base64").b64encode(np.array([1., 2., 3.], np.float32)
+                                                                            .tobytes()).decode(),
+                                  "__shape__": [3]}}}, open(f, "w"))
+    p = load_pytree(f)
+    assert type(p).__name__ == "MockPyTree171"
+    np.testing.assert_array_equal(p.x, np.array([1., 2., 3.], np.float32))
+
+
+def test_summary_fields():
+    """summary (utils.py:284-430): termination text, counts, log Z and max(log L) rounded to the
+    uncertainty, H, ESS, and per dimension mean +- std | percentiles | MAP est. | max(L) est."""
+    res = _small_results()._replace(termination_reason=4 | (1 << 10))
+    out = J.summary(res, f_obj=io.StringIO())
+    assert "Small remaining evidence" in out and "no seed points left" in out
+    assert "likelihood evals: 13" in out and "samples: 3" in out
+    assert "x[#]: mean +- std.dev. | 10%ile / 50%ile / 90%ile | MAP est. | max(L) est." in out
+    # MAP point is row 1 (largest log posterior), max(L) point row 2
+    line0 = [ln for ln in out.splitlines() if ln.startswith("x[0]:")][0]
+    assert line0.split("|")[-2].strip() == "2.0" and line0.split("|")[-1].strip() == "4.0"
 
 
 def test_sample_evidence_moments():
