@@ -311,6 +311,25 @@ int hipns_init_finalize(const hipns_run_desc_t* runs_dev, hipns_run_state_t* sta
  * (b) batched prior transform and likelihood
  * ------------------------------------------------------------------- */
 
+/* Lower Cholesky factor in FP64 of the symmetric positive-definite D x D matrix A_dev (row-major, lower
+ * triangle read): L_dev [D][D] (zero above the diagonal).  info_dev (may be NULL) receives 0, or k + 1
+ * when the leading minor of order k + 1 is not positive definite (L is then incomplete).  One
+ * workgroup; for the one-off factor of a likelihood covariance (jaxns users pass it to
+ * tfpd.MultivariateNormalFullCovariance / TriL, tests/conftest.py:219-224). */
+int hipns_cholesky_f64(const double* A_dev, int32_t D, double* L_dev, int32_t* info_dev, void* stream);
+
+/* The multivariate-normal family's parameters (HIPNS_FAM_MVN) for log N(x; mean, cov) on the affine
+ * prior x = loc + S q(u), formed in FP64 on the device: L = chol(cov) (into L_dev), W = L^-1 S,
+ * b = L^-1 (loc - mean), written as the FP32 block params_dev = [qkind: dps][b: dps][W: dps x dps]
+ * (zero padding), and log_const_dev[0] = -D/2 log(2 pi) - sum log L_ii.  S_dev is D x D row-major;
+ * info_dev as for hipns_cholesky_f64 (nothing else is written when it is non-zero).  cov_dev = NULL
+ * takes L_dev as the given factor (a scale_tril) and runs the solves only. */
+int hipns_mvn_prepare_f64(const double* cov_dev, const double* mean_dev, const double* loc_dev, const double* S_dev,
+                          const int32_t* qkind_dev, int32_t D, int32_t dps, double* L_dev, float* params_dev,
+                          double* log_const_dev, int32_t* info_dev, void* workspace, size_t workspace_bytes,
+                          void* stream);
+size_t hipns_mvn_prepare_workspace_size(int32_t D);
+
 /* Model.forward over a batch (framework/model.py:167-176 -> framework/ops.py:302-326):
  * logL_dev[i] = likelihood(prior_transform(U_dev[i, :D])) of run 0 of `runs_dev`,
  * NaN -> -inf.  U rows have stride bufs->dps. */

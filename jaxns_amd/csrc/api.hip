@@ -628,6 +628,33 @@ int hipns_slice_quotients(const float* a_dev, const float* n_dev, int64_t count,
   return HIPNS_OK;
 }
 
+int hipns_cholesky_f64(const double* A_dev, int32_t D, double* L_dev, int32_t* info_dev, void* stream) {
+  if (D < 1 || D > HIPNS_MAX_DIM) return fail(HIPNS_EINVAL, "D outside [1, %d]", HIPNS_MAX_DIM);
+  if (!A_dev || !L_dev) return fail(HIPNS_EINVAL, "null pointer");
+  hipError_t e = hipns::cholesky_f64(A_dev, D, L_dev, info_dev, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "k_cholesky");
+  return HIPNS_OK;
+}
+
+size_t hipns_mvn_prepare_workspace_size(int32_t D) { return D < 1 ? 0 : hipns::mvn_prepare_ws(D); }
+
+int hipns_mvn_prepare_f64(const double* cov_dev, const double* mean_dev, const double* loc_dev, const double* S_dev,
+                          const int32_t* qkind_dev, int32_t D, int32_t dps, double* L_dev, float* params_dev,
+                          double* log_const_dev, int32_t* info_dev, void* workspace, size_t workspace_bytes,
+                          void* stream) {
+  if (D < 1 || D > HIPNS_MAX_DIM) return fail(HIPNS_EINVAL, "D outside [1, %d]", HIPNS_MAX_DIM);
+  if (dps < D || dps % 32 || dps > HIPNS_MAX_DIM) return fail(HIPNS_EINVAL, "dps must be a multiple of 32 in [D, %d]",
+                                                              HIPNS_MAX_DIM);
+  if (!mean_dev || !loc_dev || !S_dev || !qkind_dev || !L_dev || !params_dev || !log_const_dev)
+    return fail(HIPNS_EINVAL, "null pointer");
+  if (!workspace || workspace_bytes < hipns::mvn_prepare_ws(D))
+    return fail(HIPNS_ENOMEM, "workspace of %zu bytes needed", hipns::mvn_prepare_ws(D));
+  hipError_t e = hipns::mvn_prepare_f64(cov_dev, mean_dev, loc_dev, S_dev, qkind_dev, D, dps, L_dev, workspace,
+                                        params_dev, log_const_dev, info_dev, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "k_mvn_prepare");
+  return HIPNS_OK;
+}
+
 int hipns_results_finals(const double* log_dZ_dev, const double* log_L_dev, const double* log_X_dev, int64_t M,
                          double log_Z_mean, double* log_dp_dev, double* H_dev, void* workspace, size_t workspace_bytes,
                          void* stream) {

@@ -33,6 +33,13 @@ size_t batch_evidence_ws(int R, int64_t max_M);
 hipError_t batch_evidence(const hipns_run_desc_t* runs, const hipns_run_state_t* states, int R, int64_t max_M,
                           const hipns_buffers_t& B, double* fin, int64_t* phantom, void* ws, hipStream_t s);
 
+// one-off FP64 preparation of the multivariate-normal likelihood (ns_prep.hip)
+hipError_t cholesky_f64(const double* A, int D, double* L, int* info, hipStream_t s);
+size_t mvn_prepare_ws(int D);
+hipError_t mvn_prepare_f64(const double* C, const double* mu, const double* loc, const double* S,
+                           const int32_t* qkind, int D, int dps, double* L, void* ws, float* prm, double* log_const,
+                           int* info, hipStream_t s);
+
 // _to_results finals (ns_finals.hip)
 size_t results_finals_ws(int64_t M);
 hipError_t results_finals(const double* log_dZ, const double* log_L, const double* log_X, int64_t M,

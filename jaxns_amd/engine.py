@@ -149,7 +149,7 @@ class DeviceBatch:
                 p, c = s.model.log_likelihood.device_params(s.model, dps)
             prms.append(p)
             consts.append(c)
-        poffs = np.cumsum([0] + [p.size for p in prms])
+        poffs = np.cumsum([0] + [int(p.numel()) if torch.is_tensor(p) else int(p.size) for p in prms])
         lr_ndata = lr_panels = lr_off = 0
         if self.family == _lib.FAM_LINREG:
             data = fam.data
@@ -159,7 +159,11 @@ class DeviceBatch:
             lr_off = int(-(-poffs[-1] // 64) * 64)  # 256-byte aligned start of the shared rows
             prms.append(np.zeros(lr_off - poffs[-1], np.float32))
             prms.append(blk)
-        self.params = torch.from_numpy(np.concatenate(prms)).to(dev)
+        # parameter blocks formed on the host (numpy) or on the device (a tensor, e.g. the FP64-prepared
+        # multivariate normal) are packed back to back
+        self.params = torch.cat([p.to(device=dev, dtype=torch.float32).reshape(-1) if torch.is_tensor(p)
+                                 else torch.from_numpy(np.ascontiguousarray(p, dtype=np.float32)).to(dev)
+                                 for p in prms])
         # descriptors
         arr = (RunDesc * self.R)()
         lo = so = 0

@@ -34,18 +34,32 @@ class LikelihoodFamily:
 
 
 class MultivariateNormalLikelihood(LikelihoodFamily):
-    """log N(x; mean, cov) (tests/conftest.py:219-224)."""
+    """log N(x; mean, cov) (tests/conftest.py:219-224).
+
+    Given a covariance, its Cholesky factor is formed once on the GPU in FP64 (hipns_cholesky_f64);
+    the sampling kernel's parameters (W = L^-1 S, b = L^-1 (loc - mean) for the model's affine prior)
+    come from the same factor by FP64 solves on the GPU (hipns_mvn_prepare_f64)."""
 
     family = FAM_MVN
 
     def __init__(self, mean, cov=None, scale_tril=None):
         self.mean = np.atleast_1d(np.asarray(mean, dtype=np.float64))
-        if scale_tril is None:
-            cov = np.asarray(cov, dtype=np.float64)
-            scale_tril = np.linalg.cholesky(cov)
-        self.scale_tril = np.tril(np.asarray(scale_tril, dtype=np.float64))
         self.D = self.mean.size
-        self.log_const = -0.5 * self.D * np.log(2 * np.pi) - np.sum(np.log(np.diag(self.scale_tril)))
+        if cov is None and scale_tril is None:
+            raise ValueError("give cov or scale_tril")
+        self.cov = None if cov is None else np.asarray(cov, dtype=np.float64).reshape(self.D, self.D)
+        self._scale_tril = None if scale_tril is None else np.tril(np.asarray(scale_tril, dtype=np.float64))
+        self._prepared = {}
+
+    @property
+    def scale_tril(self) -> np.ndarray:
+        if self._scale_tril is None:
+            self._scale_tril = device_cholesky(self.cov)
+        return self._scale_tril
+
+    @property
+    def log_const(self) -> float:
+        return float(-0.5 * self.D * np.log(2 * np.pi) - np.sum(np.log(np.diag(self.scale_tril))))
 
     def log_prob_numpy(self, X):
         from scipy.linalg import solve_triangular
@@ -62,20 +76,62 @@ class MultivariateNormalLikelihood(LikelihoodFamily):
         return self.log_const - 0.5 * (z * z).sum(0)
 
     def device_params(self, model, dps):
-        from scipy.linalg import solve_triangular
+        """(FP32 parameter block on the GPU, log_const), formed once per (model, dps) by
+        hipns_mvn_prepare_f64."""
         D = model.U_ndims
         if D != self.D:
             raise ValueError(f"likelihood dimension {self.D} != model U dimension {D}")
-        qkind, loc, S = model.prior_program()
-        W = solve_triangular(self.scale_tril, S, lower=True)
-        b = solve_triangular(self.scale_tril, loc - self.mean, lower=True)
-        prm = np.zeros(2 * dps + dps * dps, np.float32)
-        prm[:D] = qkind
-        prm[dps:dps + D] = b
-        Wp = np.zeros((dps, dps))
-        Wp[:D, :D] = W
-        prm[2 * dps:] = Wp.ravel()
-        return prm, float(self.log_const)
+        key = (id(model), dps)
+        if key not in self._prepared:
+            prm, c, L = mvn_prepare_device(self, model, dps)
+            if self._scale_tril is None:
+                self._scale_tril = L
+            self._prepared[key] = (model, prm, c)
+        _, prm, c = self._prepared[key]
+        return prm, c
+
+
+def device_cholesky(A: np.ndarray) -> np.ndarray:
+    """Lower Cholesky factor of A in FP64 on the GPU (hipns_cholesky_f64)."""
+    import torch
+    from ._lib import check, lib, ptr, stream_ptr
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    D = A.shape[0]
+    Ad = torch.from_numpy(A).cuda()
+    Ld = torch.empty_like(Ad)
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    check(lib().hipns_cholesky_f64(ptr(Ad), D, ptr(Ld), ptr(info), stream_ptr()), "hipns_cholesky_f64")
+    if int(info.item()):
+        raise np.linalg.LinAlgError(f"covariance is not positive definite (leading minor {int(info.item())})")
+    return Ld.cpu().numpy()
+
+
+def mvn_prepare_device(lik: "MultivariateNormalLikelihood", model, dps: int):
+    """hipns_mvn_prepare_f64 for the model's affine prior: (params [2 dps + dps^2] float32 on the GPU,
+    log_const, L as a host array)."""
+    import torch
+    from ._lib import check, lib, ptr, stream_ptr
+    D = lik.D
+    qkind, loc, S = model.prior_program()
+    f64 = dict(dtype=torch.float64, device="cuda")
+    mu = torch.as_tensor(lik.mean, **f64)
+    locd = torch.as_tensor(np.ascontiguousarray(loc), **f64)
+    Sd = torch.as_tensor(np.ascontiguousarray(S), **f64)
+    qk = torch.as_tensor(np.ascontiguousarray(qkind, dtype=np.int32), device="cuda")
+    if lik.cov is not None and lik._scale_tril is None:
+        cov, Ld = torch.as_tensor(lik.cov, **f64), torch.empty((D, D), **f64)
+    else:
+        cov, Ld = None, torch.as_tensor(np.ascontiguousarray(lik.scale_tril), **f64)
+    prm = torch.empty(2 * dps + dps * dps, dtype=torch.float32, device="cuda")
+    c = torch.empty(1, **f64)
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = torch.empty(int(lib().hipns_mvn_prepare_workspace_size(D)), dtype=torch.uint8, device="cuda")
+    check(lib().hipns_mvn_prepare_f64(None if cov is None else ptr(cov), ptr(mu), ptr(locd), ptr(Sd), ptr(qk), D, dps,
+                                      ptr(Ld), ptr(prm), ptr(c), ptr(info), ptr(ws), ws.numel(), stream_ptr()),
+          "hipns_mvn_prepare_f64")
+    if int(info.item()):
+        raise np.linalg.LinAlgError(f"covariance is not positive definite (leading minor {int(info.item())})")
+    return prm, float(c.item()), Ld.cpu().numpy()
 
 
 class GaussianMixtureLikelihood(LikelihoodFamily):

@@ -97,28 +97,15 @@ def test_model_parsing_and_transform():
         J.Model(bad, J.MultivariateNormalLikelihood(np.zeros(2), cov=np.eye(2)))
 
 
-def test_mvn_device_params_reproduce_likelihood():
-    """The packed (W, b, c) satisfy c - |W q(u) + b|^2/2 = log N(x(u); mu, Sigma) in FP64."""
-    def prior_model():
-        x = yield J.Prior(tfpd.MultivariateNormalTriL(loc=np.full(3, 15.0),
-                                                      scale_tril=np.array([[1, 0, 0], [.5, 1, 0], [.2, .3, 2.]])))
-        return x
-
-    Sig = np.full((3, 3), 0.99) + 0.01 * np.eye(3)
-    lik = J.MultivariateNormalLikelihood(np.zeros(3), cov=Sig)
-    m = J.Model(prior_model, lik)
-    prm, c = lik.device_params(m, 32)
-    q = prm[:32]
-    b = prm[32:64].astype(np.float64)
-    W = prm[64:].reshape(32, 32).astype(np.float64)
-    rng = np.random.default_rng(0)
-    U = rng.uniform(0.05, 0.95, size=(50, 3))
-    z = np.zeros((50, 32))
-    z[:, :3] = np.where(q[:3] == 1, ndtri(U), U)
-    r = z @ W.T + b
-    got = c - 0.5 * np.sum(r * r, axis=1)
-    want = m.forward_numpy(U)
-    np.testing.assert_allclose(got, want, rtol=1e-5)
+def test_mvn_likelihood_defers_its_factor_to_the_device():
+    """A multivariate-normal likelihood given a covariance keeps it: the Cholesky factor is formed on the
+    GPU in FP64 when first needed, so building the model needs no device; a given scale_tril is kept."""
+    lik = J.MultivariateNormalLikelihood(np.zeros(3), cov=np.eye(3) * 2.0)
+    assert lik._scale_tril is None and lik.cov.shape == (3, 3)
+    L = np.array([[1.0, 0, 0], [0.5, 2.0, 0], [0.1, 0.2, 3.0]])
+    lik2 = J.MultivariateNormalLikelihood(np.zeros(3), scale_tril=L)
+    np.testing.assert_array_equal(lik2.scale_tril, L)
+    assert lik2.log_const == pytest.approx(-1.5 * np.log(2 * np.pi) - np.log(6.0))
 
 
 def test_gmm_params_and_analytic_evidence():
