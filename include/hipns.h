@@ -218,6 +218,16 @@ int hipns_evidence_scan(const double* log_L_dev, const double* num_live_points_d
                         void* stream);
 size_t hipns_evidence_scan_workspace_size(int64_t M);
 
+/* hipns_evidence_scan for callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): no counterpart in the reference.  The same outputs, with the scan's
+ * scratch in a fixed 32 MiB device array inside the library (one per GPU, held until the process ends);
+ * M is limited to hipns_evidence_scan_noalloc_max() samples (HIPNS_ERANGE beyond it).  Nothing is
+ * allocated.  Calls that overlap on one GPU share that array and race: serialise them (one stream, or
+ * stream-ordered). */
+int hipns_evidence_scan_noalloc(const double* log_L_dev, const double* num_live_points_dev, int64_t M,
+                                int64_t stop_idx, double* per_sample_dev, double* final_dev, void* stream);
+size_t hipns_evidence_scan_noalloc_max(void);
+
 /* count_crossed_edges (internals/tree_structure.py:33-108): stable sort of the
  * N+1 node log-likelihoods (root -inf), out-degree by sender, prefix count.
  * num_samples < 0 => static form. */

@@ -143,6 +143,9 @@ def _declare(L):
                                         _vp]
     L.hipns_mvn_prepare_workspace_size.restype = C.c_size_t
     L.hipns_mvn_prepare_workspace_size.argtypes = [_i32]
+    L.hipns_evidence_scan_noalloc.argtypes = [_vp, _vp, _i64, _i64, _vp, _vp, _vp]
+    L.hipns_evidence_scan_noalloc_max.restype = C.c_size_t
+    L.hipns_evidence_scan_noalloc_max.argtypes = []
     L.hipns_results_finals.argtypes = [_vp, _vp, _vp, _i64, _d, _vp, _vp, _vp, C.c_size_t, _vp]
     L.hipns_results_finals_workspace_size.restype = C.c_size_t
     L.hipns_results_finals_workspace_size.argtypes = [_i64]
@@ -161,7 +164,7 @@ EXPORTED = ["hipns_abi_version", "hipns_last_error", "hipns_create_init_state", 
             "hipns_prior_transform", "hipns_slice_chains_injected", "hipns_philox4x32",
             "hipns_ndtri_f32", "hipns_slice_quotients", "hipns_seed_indices", "hipns_slice_directions",
             "hipns_results_finals", "hipns_results_finals_workspace_size", "hipns_prior_finals",
-            "hipns_cholesky_f64",
+            "hipns_evidence_scan_noalloc", "hipns_evidence_scan_noalloc_max", "hipns_cholesky_f64",
             "hipns_mvn_prepare_f64", "hipns_mvn_prepare_workspace_size"]
 
 

@@ -655,6 +655,22 @@ int hipns_mvn_prepare_f64(const double* cov_dev, const double* mean_dev, const d
   return HIPNS_OK;
 }
 
+int hipns_evidence_scan_noalloc(const double* log_L_dev, const double* num_live_points_dev, int64_t M,
+                                int64_t stop_idx, double* per_sample_dev, double* final_dev, void* stream) {
+  if (M < 0) return fail(HIPNS_EINVAL, "M < 0");
+  if (M > 0 && (!log_L_dev || !num_live_points_dev)) return fail(HIPNS_EINVAL, "null input");
+  if (!final_dev && !per_sample_dev) return fail(HIPNS_EINVAL, "no output");
+  if ((size_t)M > hipns::evidence_scan_static_max())
+    return fail(HIPNS_ERANGE, "M=%lld beyond the %zu samples the library's fixed scan workspace holds; use "
+                "hipns_evidence_scan with a caller workspace", (long long)M, hipns::evidence_scan_static_max());
+  hipError_t e = hipns::evidence_scan_static(log_L_dev, num_live_points_dev, M, stop_idx, per_sample_dev, final_dev,
+                                             as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "evidence scan (fixed workspace)");
+  return HIPNS_OK;
+}
+
+size_t hipns_evidence_scan_noalloc_max(void) { return hipns::evidence_scan_static_max(); }
+
 int hipns_results_finals(const double* log_dZ_dev, const double* log_L_dev, const double* log_X_dev, int64_t M,
                          double log_Z_mean, double* log_dp_dev, double* H_dev, void* workspace, size_t workspace_bytes,
                          void* stream) {

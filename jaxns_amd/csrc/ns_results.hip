@@ -1,5 +1,6 @@
 // ns_results.hip -- to_results primitives: the evidence scan over all samples,
 // a stable FP64 argsort, and count_crossed_edges.
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -278,6 +279,50 @@ hipError_t evidence_scan(const double* logL, const double* n, int64_t M, int64_t
   if (M == 0) return hipGetLastError();
   ev_launch(EvSegFlat{logL, n, M, stop}, 1, M, ws, per8, stop > 0 ? fin8 : nullptr, s);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The evidence scan without a caller workspace: its tile aggregates and per-thread prefixes live in a
+// file-scope device array, one copy per GPU for the life of the process.  Every element the scan
+// reads there is written by the same call first (k_ev_reduce writes every aggregate and prefix,
+// k_ev_scan_tiles rewrites the aggregates in place), so nothing needs clearing between calls.  Calls
+// that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[HIPNS_SCAN_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_dev[HIPNS_MAX_DEVICES] = {};
+}  // namespace
+
+size_t evidence_scan_static_max() {
+  // the largest M whose workspace fits the array
+  const size_t per_tile = sizeof(EvMap) * (1 + EV_THREADS);
+  return (size_t)((HIPNS_SCAN_SCRATCH_BYTES - 256) / per_tile) * EV_TILE;
+}
+
+hipError_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= HIPNS_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+hipError_t evidence_scan_static(const double* logL, const double* n, int64_t M, int64_t stop, double* per8,
+                                double* fin8, hipStream_t s) {
+  void* ws = nullptr;
+  hipError_t e = scratch_address(&ws);
+  if (e != hipSuccess) return e;
+  return evidence_scan(logL, n, M, stop, per8, fin8, ws, HIPNS_SCAN_SCRATCH_BYTES, s);
 }
 
 // ===========================================================================

@@ -24,6 +24,12 @@ size_t replace_lds_bytes(int N);
 hipError_t evidence_scan(const double* logL, const double* n, int64_t M, int64_t stop, double* per8, double* fin8,
                          void* ws, size_t ws_bytes, hipStream_t s);
 size_t evidence_scan_ws(int64_t M);
+// the same scan with its workspace in a file-scope device array of HIPNS_SCAN_SCRATCH_BYTES (no allocation)
+#define HIPNS_SCAN_SCRATCH_BYTES (32u << 20)
+#define HIPNS_MAX_DEVICES 64
+size_t evidence_scan_static_max();
+hipError_t evidence_scan_static(const double* logL, const double* n, int64_t M, int64_t stop, double* per8,
+                                double* fin8, hipStream_t s);
 hipError_t argsort_stable(const double* keys, int64_t n, int32_t* perm, void* ws, size_t ws_bytes, hipStream_t s);
 size_t argsort_ws(int64_t n);
 hipError_t count_crossed_edges(const int32_t* sender, const double* logL, int64_t N, int64_t num_samples,
