@@ -39,11 +39,34 @@ PEAK_FP32_MFMA = 157.3e12
 
 # HBM bytes per k_sample launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the same
 # instrumented batch (counters scaled by a 256 MiB copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
-# profiles/k_sample_mvn_d32_n2048_r512.md and profiles/k_sample_linreg_4096x16_n1024_r128.md.
+# profiles/k_sample_mvn_d32_n2048_r1024.md, profiles/k_sample_mvn_d32_n2048_r512.md and
+# profiles/k_sample_linreg_4096x16_n1024_r128.md.
 # Keyed by (workload, runs per launch); other shapes report null.
 MEASURED_TRAFFIC = {("c2", 512): 184.95e6, ("c2", 1024): 369.82e6, ("c4", 128): 17.68e6, ("c4", 256): 32.55e6}
 PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
+# wavefront VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 FP32 instruction
+# per 2 cycles per SIMD (the 157.3 TFLOP/s FP32 vector peak is 32 FMA lanes per SIMD per cycle)
+PEAK_VALU_ISSUE = 256 * 4 * 2.4e9 / 2.0
+# SQ_INSTS_VALU per k_sample launch (rocprofv3 --pmc on the bench's instrumented pass of the same
+# shape), keyed like MEASURED_TRAFFIC
+MEASURED_VALU = {("c2", 512): 6.644e9, ("c2", 1024): 1.3336e10}
+# the oracle ensemble of the same workload (tests/golden/oracle_logZ_ensembles.json) beside the
+# device's log Z statistics
+# runs of a step streamed through this many device slots (engine.run_queue) instead of one batch
+QUEUE_SLOTS = {"c3": 512}
+ORACLE_ENSEMBLE_KEY = {"c1": "mvn_d10", "c2": "mvn_d32", "c4": "linreg", "c3": "gmm"}
+
+
+def oracle_ensemble(name):
+    """Summary of the oracle's ensemble of this workload from the committed fixture (or None)."""
+    path = os.path.join(ROOT, "tests", "golden", "oracle_logZ_ensembles.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return dict(d[name]["summary"], source="tests/golden/oracle_logZ_ensembles.json")
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 # ----------------------------------------------------------------------------- workloads
@@ -425,6 +448,44 @@ def precision_at_scale(wl, R, shells, lpc, wpb, steps):
     return out
 
 
+def full_runs_at_scale(wl, R, wpb, graph_iters):
+    """R whole runs of the 65 536 x 128 regression to termination in one batch (BF16x3 acceptance, accepted
+    points refined in plain FP32 before they enter the live set), one chain group per workgroup so that
+    even a few runs spread over every CU; log Z of each run against the analytic evidence."""
+    import torch
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    specs = [RunSpec(model=wl.model, num_live_points=wl.N, num_slices=wl.D * wl.s, k_phantom=0,
+                     max_samples=wl.N * 1600, seed=70_000 + i) for i in range(R)]
+    b = DeviceBatch(specs, lanes_per_chain=4, waves_per_block=wpb, max_steps=512, lr_terms=3, refine_fp32=True)
+    term = dict(dlogZ=math.log(1.0 + 1e-3), max_samples=float(specs[0].max_samples))
+    t0 = time.perf_counter()
+    b.init_state()
+    b.check_termination(dict(efficiency_threshold=0.1, dlogZ=0.0, max_samples=term["max_samples"]), True)
+    b.check_termination(term, True)
+    shells = 0
+    while not np.all(b.done_flags()):
+        for _ in range(graph_iters):
+            b.collect_shell(term)
+        shells += graph_iters
+        torch.cuda.synchronize()
+        st = b.states()
+        log(f"c5 full runs: {shells} shells, {time.perf_counter() - t0:.0f} s, contour {st[0].log_L_contour:.6g}, "
+            f"evals {sum(x.num_likelihood_evaluations for x in st):.4g}")
+    b.consume()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    z = b.log_Z()
+    sts = b.states()
+    evals = sum(x.num_likelihood_evaluations for x in sts)
+    dz = z[:, 0] - wl.log_Z
+    return dict(runs=R, wall_s=wall, runs_per_s=R / wall, evals_per_s=evals / wall, evals_per_run=evals / R,
+                shells=[int(x.iteration) for x in sts], log_Z=[float(v) for v in z[:, 0]],
+                log_Z_uncert=[float(v) for v in z[:, 1]], log_Z_true=float(wl.log_Z), dlogZ=[float(v) for v in dz],
+                z_scores=[float(v) for v in dz / z[:, 1]], cap_hits=int(b.cap_hits().sum()),
+                refined_below_contour=[int(v) for v in b.refine_below.cpu().numpy()],
+                termination_reason=[int(x.termination_reason) for x in sts], waves_per_block=wpb)
+
+
 def to_results_scale(M, iters=5):
     """The post-path primitives of _to_results at M samples, each timed with HIP events on the stream it
     runs on: the evidence scan (compute_evidence_stats, 80 algorithmic bytes per sample: read log L and n,
@@ -480,6 +541,101 @@ def to_results_scale(M, iters=5):
     return dict(to_results_scale=True, M=M, iters=iters, kernels=out, argsort_matches_torch=ok)
 
 
+def run_step_queue(wl, seeds, lpc, wpb, slots, graph_iters):
+    """One step of the chosen form through engine.run_queue: the runs stream through `slots` device slots;
+    returns ([R, 2] log Z, truths, evaluations, step-cap hits)."""
+    from jaxns_amd.engine import run_queue
+    specs, truths = wl.specs(seeds)
+    res = run_queue(specs, slots, dict(dlogZ=math.log(1.0 + 1e-3), max_samples=float(max(s.max_samples for s in specs))),
+                    graph_iters=graph_iters, lanes_per_chain=lpc, waves_per_block=wpb, max_steps=512,
+                    lr_terms=wl.lr_terms)
+    z = np.array([(r["log_Z_mean"], r["log_Z_uncert"]) for r in res])
+    return z, truths, sum(r["num_likelihood_evaluations"] for r in res), sum(r["cap_hits"] for r in res)
+
+
+def measure_chosen(wl, cfg, R, G, steps, warmup, graph_iters, lpc, wpb, tune, seed_base=0, slots=0):
+    """The chosen form on one workload: `warmup` untimed then `steps` timed steps of R runs (G concurrent
+    groups, or streamed through `slots` slots when slots > 0), each from create_init_state to log Z.
+    Returns the step statistics."""
+    import torch
+    if tune:
+        from jaxns_amd.autotune import autotune
+        t0 = time.perf_counter()
+        RG = (R + G - 1) // G
+        specs0, _ = wl.specs(range(seed_base, seed_base + RG))
+        lpc, wpb = autotune(specs0, verbose=True, lr_terms=wl.lr_terms)
+        log(f"{cfg}: autotune picked lanes_per_chain={lpc} waves_per_block={wpb} in {time.perf_counter() - t0:.1f} s")
+    for w in range(warmup):
+        seeds = range(1_000_000 + w * R, 1_000_000 + (w + 1) * R)
+        t0 = time.perf_counter()
+        if slots:
+            run_step_queue(wl, seeds, lpc, wpb, slots, graph_iters)
+        else:
+            bs, _ = make_groups(wl, seeds, lpc, wpb, G)
+            run_groups(bs, graph_iters)
+            for b in bs:
+                b.log_Z()
+            del bs
+        log(f"{cfg} warmup {w}: {R} runs in {time.perf_counter() - t0:.2f} s")
+    torch.cuda.synchronize()
+    logZ, truths = [], []
+    evals_total = 0
+    caps = 0
+    t_total = 0.0
+    for k in range(steps):
+        seeds = range(seed_base + k * R, seed_base + (k + 1) * R)
+        if slots:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            z, tr, ev, cp = run_step_queue(wl, seeds, lpc, wpb, slots, graph_iters)
+            t1 = time.perf_counter()
+            evals_total += ev
+            caps += cp
+        else:
+            bs, tr = make_groups(wl, seeds, lpc, wpb, G)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run_groups(bs, graph_iters)
+            t1 = time.perf_counter()
+            z = np.concatenate([b.log_Z() for b in bs])
+            evals_total += sum(st.num_likelihood_evaluations for b in bs for st in b.states())
+            caps += int(sum(b.cap_hits().sum() for b in bs))
+            del bs
+        torch.cuda.synchronize()
+        t_total += time.perf_counter() - t0
+        logZ.append(z)
+        truths.append(tr)
+        log(f"{cfg} step {k}: {R} runs, sampling {t1 - t0:.3f} s, log Z {time.perf_counter() - t1:.3f} s, total "
+            f"{t_total:.2f} s, mean |dlogZ| {np.abs(z[:, 0] - tr).mean():.3f}")
+    logZ = np.concatenate(logZ)
+    truths = np.concatenate(truths)
+    dz = logZ[:, 0] - truths
+    return dict(runs_per_s=R * steps / t_total, ms_per_step=1e3 * t_total / steps, t_total=t_total,
+                evals_per_s=evals_total / t_total, evals_per_run=evals_total / (R * steps), cap_hits=caps,
+                mean_dlogZ=float(dz.mean()), se_mean_dlogZ=float(dz.std(ddof=1) / math.sqrt(dz.size)) if dz.size > 1
+                else None, mean_abs_dlogZ=float(np.abs(dz).mean()), frac_abs_dlogZ_lt_0_1=float(np.mean(np.abs(dz) < 0.1)),
+                mean_stated_uncert=float(logZ[:, 1].mean()), frac_z_lt_3=float(np.mean(np.abs(dz) / logZ[:, 1] < 3)),
+                ensemble_std_logZ=float(logZ[:, 0].std()), lpc=lpc, wpb=wpb, logZ=logZ, truths=truths)
+
+
+def secondary(cfg, R, steps, graph_iters, slots=0):
+    """One further configuration measured in the same process (autotuned launch shape, `steps` timed
+    steps of R runs, no warm-up): its runs/s, evaluations/s, log Z statistics and step-cap hits, and
+    the end-to-end algorithmic rate of the step against the roofline of its dominant kernel."""
+    wl = make_workload(cfg)
+    m = measure_chosen(wl, cfg, R, 1, steps, 0, graph_iters, 4, 4, True, slots=slots)
+    out = {k: v for k, v in m.items() if k not in ("logZ", "truths", "t_total")}
+    specs, _ = wl.specs(range(1))
+    work = m["evals_per_s"] * wl.work_per_eval(specs[0]) if cfg != "c3" else None
+    out.update(workload=wl.name, runs_per_step=R, steps=steps, warmup=0, dtype=wl.dtype, queue_slots=slots)
+    if work is not None:
+        scale = 1e12 if wl.unit == "TFLOP/s" else 1e9
+        out["roofline_end_to_end"] = dict(bound=wl.bound, achieved=work / scale, peak=wl.peak / scale, unit=wl.unit,
+                                          frac=work / wl.peak, kernel=wl.kernel,
+                                          note="algorithmic work of the whole timed step over its wall time")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -489,15 +645,21 @@ def main():
                     help="runs per step (chosen form); default c2: 1024, c1: 128, c3: 1024, c4: 256, c5: 8")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--shells", type=int, default=3, help="c5: shells per partial run")
+    ap.add_argument("--full-runs", type=int, default=0, help="c5: also run this many whole runs to termination")
     ap.add_argument("--graph-iters", type=int, default=8)
-    ap.add_argument("--groups", type=int, default=0,
-                    help="chosen form: the runs of a step in this many concurrent groups, each on its own "
-                         "stream (default c2, c3 and c4: 2, others 1)")
+    ap.add_argument("--groups", type=int, default=1,
+                    help="chosen form: the runs of a step in this many concurrent groups, each on its own stream "
+                         "(the roofline's instrumented launches then cover one group)")
+    ap.add_argument("--slots", type=int, default=-1,
+                    help="stream the runs of a step through this many device slots (engine.run_queue); "
+                         "0 = one batch; default: c3 %d, others 0" % 512)
     ap.add_argument("--lpc", type=int, default=4)
     ap.add_argument("--wpb", type=int, default=4)
     ap.add_argument("--lr-terms", type=int, default=3, choices=[3, 4])
     ap.add_argument("--plain-runs", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the one-step measurements of the other configurations")
     ap.add_argument("--no-autotune", action="store_true", help="use --lpc/--wpb instead of the autotuner")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only the instrumented shell-by-shell pass (for rocprofv3 kernel statistics)")
@@ -528,20 +690,17 @@ def main():
     wl = make_workload(args.config)
     if args.config == "c4":
         wl.lr_terms = args.lr_terms
-    # c2: 1024 runs per step (512: 920 runs/s, 768: 939, 1024: 952 on one box; the shell kernel's tail
-    # and the per-run kernels amortise over more runs)
-    # c3: 1024 (256: 54.7 runs/s, 512: 65.9, 1024: 74.9; the lock-step shells' tail of the longest runs
-    # amortises over more runs)
+    # runs per step: c2 1024 (512: 920 runs/s, 768: 939, 1024: 952 on one box; the shell kernel's tail and
+    # the per-run kernels amortise over more runs); c3 1024 (256: 54.7, 512: 65.9, 1024: 74.9)
     R = args.runs or {"c2": 1024, "c3": 1024, "c4": 256, "c5": 8}.get(args.config, 128)
-    # two groups on their own streams: c2 975 against 953 runs/s, c3 81.0 against 74.9 (four: 81.5),
-    # c4 68.8 against 64.0
-    G = max(1, args.groups if args.groups else {"c2": 2, "c3": 2, "c4": 2}.get(args.config, 1))
+    G = max(1, args.groups)
     RG = (R + G - 1) // G  # runs per group: the batch one launch of the dominant kernel covers
     seed_base = 0
     if args.config == "c5":
         t0 = time.perf_counter()
         res = precision_at_scale(wl, R, args.shells, 4, args.wpb, args.steps)
         wall = time.perf_counter() - t0
+        full = full_runs_at_scale(wl, args.full_runs, 1, args.graph_iters) if args.full_runs else None
         bf, fp = res["bf16x3"], res["fp32"]
         roof = dict(bound="mfma", achieved=bf["achieved_TFLOPs"], peak=PEAK_BF16_MFMA / 1e12, unit="TFLOP/s",
                     frac=bf["achieved_TFLOPs"] * 1e12 / PEAK_BF16_MFMA, traffic=None, kernel=wl.kernel,
@@ -554,7 +713,8 @@ def main():
             "config": {"workload": wl.name, "runs_per_step": R, "shells": args.shells, "lanes_per_chain": 4,
                        "waves_per_block": args.wpb},
             "bf16x3": bf, "plain_fp32": fp, "bf16x3_vs_fp32": bf["evals_per_s"] / fp["evals_per_s"],
-            "log_Z_true": float(wl.log_Z), "wall_s": wall, "roofline": roof, "cpu_baseline": None}), flush=True)
+            "log_Z_true": float(wl.log_Z), "wall_s": wall, "roofline": roof, "cpu_baseline": None,
+            "full_runs": full}), flush=True)
         return
     if args.profile_pass:
         durs, work, evs = instrumented_slice_timing(wl, R, args.lpc, args.wpb, seed0=90000)
@@ -564,49 +724,15 @@ def main():
                           "achieved": float(work.mean() / durs.mean()), "unit": wl.unit[:-2] + "/s",
                           "lane_utilisation": instrumented_slice_timing.lane_util}), flush=True)
         return
-    if not args.no_autotune:
-        from jaxns_amd.autotune import autotune
-        t0 = time.perf_counter()
-        specs0, _ = wl.specs(range(seed_base, seed_base + RG))
-        args.lpc, args.wpb = autotune(specs0, verbose=True, lr_terms=wl.lr_terms)
-        log(f"autotune picked lanes_per_chain={args.lpc} waves_per_block={args.wpb} "
-            f"in {time.perf_counter() - t0:.1f} s")
 
     # ---------------- chosen form: R runs per step
-    for w in range(args.warmup):
-        bs, _ = make_groups(wl, range(1000 + w * R, 1000 + (w + 1) * R), args.lpc, args.wpb, G)
-        t0 = time.perf_counter()
-        run_groups(bs, args.graph_iters)
-        for b in bs:
-            b.log_Z()
-        log(f"warmup {w}: {R} runs in {G} group(s) in {time.perf_counter() - t0:.2f} s")
-        del bs
-    torch.cuda.synchronize()
-    logZ, truths = [], []
-    evals_total = 0
-    t_total = 0.0
-    for k in range(args.steps):
-        bs, tr = make_groups(wl, range(seed_base + k * R, seed_base + (k + 1) * R), args.lpc, args.wpb, G)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        run_groups(bs, args.graph_iters)
-        t1 = time.perf_counter()
-        z = np.concatenate([b.log_Z() for b in bs])
-        torch.cuda.synchronize()
-        t_total += time.perf_counter() - t0
-        log(f"step {k}: sampling {t1 - t0:.3f} s, log Z of {R} runs {time.perf_counter() - t1:.3f} s")
-        logZ.append(z)
-        truths.append(tr)
-        evals_total += sum(st.num_likelihood_evaluations for b in bs for st in b.states())
-        log(f"step {k}: {R} runs, total {t_total:.2f} s, mean |dlogZ| {np.abs(z[:, 0] - tr).mean():.3f}")
-        del bs
-    logZ = np.concatenate(logZ)
-    truths = np.concatenate(truths)
-    ms_per_step = 1e3 * t_total / args.steps
-    runs_per_s = R * args.steps / t_total
-    evals_per_s = evals_total / t_total
-    dz = logZ[:, 0] - truths
-    evals_per_run = evals_total / (R * args.steps)
+    slots = QUEUE_SLOTS.get(args.config, 0) if args.slots < 0 else args.slots
+    m = measure_chosen(wl, args.config, R, G, args.steps, args.warmup, args.graph_iters, args.lpc, args.wpb,
+                       not args.no_autotune, seed_base, slots=slots)
+    args.lpc, args.wpb = m["lpc"], m["wpb"]
+    ms_per_step = m["ms_per_step"]
+    runs_per_s = m["runs_per_s"]
+    dz = m["logZ"][:, 0] - m["truths"]
 
     # ---------------- plain form: one run at a time, host sync per shell, FP32 likelihood GEMM
     tp = 0.0
@@ -620,27 +746,43 @@ def main():
         tp += time.perf_counter() - t0
         log(f"plain run {k}: {tp:.2f} s so far")
         del b
-    plain_runs_per_s = args.plain_runs / tp
+    plain_runs_per_s = args.plain_runs / tp if args.plain_runs else None
 
-    # ---------------- roofline of the dominant kernel (hipns_shell_sample), one launch covering the runs of
-    # one group (the batch size each launch of the timed chosen form has)
+    # ---------------- roofline of the dominant kernel (hipns_shell_sample): the launches of one group of
+    # the timed step (R runs per launch with the default single group), shell by shell with HIP events
     durs, work, evs = instrumented_slice_timing(wl, RG, args.lpc, args.wpb, seed0=90000)
     log(f"instrumented: {durs.size} sample launches, mean {durs.mean() * 1e3:.3f} ms")
     avg_dur = float(durs.mean())
     achieved = float(work.mean() / avg_dur)
     scale = 1e12 if wl.unit == "TFLOP/s" else 1e9
+    launches_per_step = durs.size * G
     roofline = dict(bound=wl.bound, achieved=achieved / scale, peak=wl.peak / scale, unit=wl.unit,
                     frac=achieved / wl.peak, traffic=MEASURED_TRAFFIC.get((args.config, RG)),
                     traffic_unit="bytes per launch", kernel=wl.kernel, avg_launch_ms=avg_dur * 1e3,
                     launches=int(durs.size), evals_per_launch=float(evs.mean()),
                     evals_per_s_in_kernel=float(evs.sum() / durs.sum()),
-                    lane_utilisation=instrumented_slice_timing.lane_util)
+                    lane_utilisation=instrumented_slice_timing.lane_util,
+                    kernel_ms_per_step=avg_dur * 1e3 * launches_per_step,
+                    end_to_end_achieved=m["evals_per_s"] * wl.work_per_eval(wl.specs(range(1))[0][0]) / scale)
+    valu = MEASURED_VALU.get((args.config, RG))
+    if valu is not None:
+        # the kernel's VALU issue: SQ_INSTS_VALU per launch (rocprofv3 --pmc on the same launch shape) over the
+        # live average launch duration, against one wavefront instruction per 2 cycles per SIMD
+        rate = valu / avg_dur
+        roofline["valu_ceiling"] = dict(insts_per_launch=valu, achieved=rate, peak=PEAK_VALU_ISSUE,
+                                        unit="wavefront VALU instructions/s", frac=rate / PEAK_VALU_ISSUE)
 
     cpu = None
     if not args.no_cpu:
         specs1, _ = wl.specs([seed_base])
-        cpu = cpu_baseline(wl, specs1[0], evals_per_run)
+        cpu = cpu_baseline(wl, specs1[0], m["evals_per_run"])
     log("cpu baseline", cpu)
+    sec = {}
+    if not args.no_secondary and args.config == "c2":
+        for cfg, Rs, st, sl in (("c1", 128, 2, 0), ("c4", 256, 1, 0), ("c3", 1024, 1, QUEUE_SLOTS["c3"])):
+            t0 = time.perf_counter()
+            sec[cfg] = secondary(cfg, Rs, st, args.graph_iters, slots=sl)
+            log(f"secondary {cfg}: {sec[cfg]['runs_per_s']:.2f} runs/s in {time.perf_counter() - t0:.1f} s")
     out = {
         "metric": "runs/s to |dlogZ|<0.1 nat (chosen batched form, 1 GPU)",
         "value": runs_per_s,
@@ -654,24 +796,32 @@ def main():
         "vs_baseline": None,
         "dtype": wl.dtype,
         "data": "synthetic (seeded numpy)",
-        "config": {"workload": wl.name, "runs_per_step": R, "groups": G, "graph_iters": args.graph_iters,
+        "config": {"workload": wl.name, "runs_per_step": R, "groups": G, "queue_slots": slots,
+                   "graph_iters": args.graph_iters,
                    "lanes_per_chain": args.lpc, "waves_per_block": args.wpb},
-        "likelihood_evals_per_s": evals_per_s,
-        "evals_per_run": evals_per_run,
+        "likelihood_evals_per_s": m["evals_per_s"],
+        "evals_per_run": m["evals_per_run"],
+        "cap_hits": m["cap_hits"],
         "plain_form_runs_per_s": plain_runs_per_s,
-        "chosen_vs_plain": runs_per_s / plain_runs_per_s,
+        "chosen_vs_plain": runs_per_s / plain_runs_per_s if plain_runs_per_s else None,
         "mean_dlogZ": float(dz.mean()),
+        "se_mean_dlogZ": m["se_mean_dlogZ"],
         "mean_abs_dlogZ": float(np.abs(dz).mean()),
         "frac_abs_dlogZ_lt_0.1": float(np.mean(np.abs(dz) < 0.1)),
-        "mean_stated_uncert": float(logZ[:, 1].mean()),
-        "frac_z_lt_3": float(np.mean(np.abs(dz) / logZ[:, 1] < 3)),
+        "mean_stated_uncert": m["mean_stated_uncert"],
+        "frac_z_lt_3": m["frac_z_lt_3"],
         "plain_dlogZ": [float(z) for z in pz],
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    oracle_ref = ORACLE_ENSEMBLE_KEY.get(args.config)
+    if oracle_ref:
+        out["oracle_ensemble"] = oracle_ensemble(oracle_ref)
     if args.config in ("c1", "c2", "c4"):
-        out["log_Z_true"] = float(truths[0])
-        out["ensemble_std_logZ"] = float(logZ[:, 0].std())
+        out["log_Z_true"] = float(m["truths"][0])
+        out["ensemble_std_logZ"] = m["ensemble_std_logZ"]
+    if sec:
+        out["secondary"] = sec
     print(json.dumps(out), flush=True)
 
 

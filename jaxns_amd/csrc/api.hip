@@ -76,8 +76,8 @@ int check_launch(const hipns_launch_t* L) {
     if (L->lr_panels < 2 || L->lr_panels % 2 || L->lr_panels > HIPNS_MAX_DIM / 16)
       return fail(HIPNS_EINVAL, "lr_panels must be even, 2..%d", HIPNS_MAX_DIM / 16);
     if (L->lr_data_off < 0 || L->lr_data_off % 4) return fail(HIPNS_EINVAL, "lr_data_off must be a multiple of 4");
-    if (L->lr_terms != 0 && L->lr_terms != 1 && L->lr_terms != 3 && L->lr_terms != 4)
-      return fail(HIPNS_EINVAL, "lr_terms must be 1, 3 or 4");
+    if (L->lr_terms != 0 && L->lr_terms != 1 && L->lr_terms != 3 && L->lr_terms != 4 && L->lr_terms != 5)
+      return fail(HIPNS_EINVAL, "lr_terms must be 1, 3, 4 or 5");
   }
   if (L->lanes_per_chain != 2 && L->lanes_per_chain != 4)
     return fail(HIPNS_EINVAL, "lanes_per_chain must be 2 or 4");
@@ -101,7 +101,7 @@ hipns::LinRegLds linreg_view(const hipns_buffers_t* bufs, const hipns_launch_t* 
   lr.xf = base + 2 * npad + xe;
   lr.npad = L->lr_ndata;
   lr.panels = L->lr_panels;
-  lr.terms = (L->lr_terms == 4 || L->lr_terms == 1) ? L->lr_terms : 3;
+  lr.terms = (L->lr_terms == 4 || L->lr_terms == 1 || L->lr_terms == 5) ? L->lr_terms : 3;
   return lr;
 }
 

@@ -234,9 +234,14 @@ __device__ __noinline__ float ndtri_tail_f32(float u) { return (float)ndtri_f64(
 // Giles' body for u in [1e-7, 1 - 1e-7].  w = -ln(4u(1-u)) comes from the hardware log2 (one
 // rounding more than a full-precision log: about 1e-7 relative in w, below the polynomials' own error).
 // The w >= 5 polynomial runs only when some lane of the wavefront needs it.
-__device__ __forceinline__ float ndtri_giles(float u) {
-  const float w = -0.693147180559945309f * __builtin_amdgcn_logf(4.0f * u * (1.0f - u));
-  const float x = fmaf(u, 2.82842712474619010f, -1.41421356237309505f);  // sqrt(2) (2u - 1), one rounding
+__device__ __forceinline__ float giles_w(float u) {
+  return -0.693147180559945309f * __builtin_amdgcn_logf(4.0f * u * (1.0f - u));
+}
+__device__ __forceinline__ float giles_x(float u) {
+  return fmaf(u, 2.82842712474619010f, -1.41421356237309505f);  // sqrt(2) (2u - 1), one rounding
+}
+// the central polynomial (w < 5)
+__device__ __forceinline__ float giles_central(float w) {
   const float z = w - 2.5f;
   float p = 2.81022636e-08f;
   p = fmaf(p, z, 3.43273939e-07f);
@@ -247,20 +252,29 @@ __device__ __forceinline__ float ndtri_giles(float u) {
   p = fmaf(p, z, -0.00417768164f);
   p = fmaf(p, z, 0.246640727f);
   p = fmaf(p, z, 1.50140941f);
+  return p;
+}
+// the tail polynomial (w >= 5)
+__device__ __forceinline__ float giles_tail(float w) {
+  const float v = __builtin_amdgcn_sqrtf(w) - 3.0f;
+  float pt = -0.000200214257f;
+  pt = fmaf(pt, v, 0.000100950558f);
+  pt = fmaf(pt, v, 0.00134934322f);
+  pt = fmaf(pt, v, -0.00367342844f);
+  pt = fmaf(pt, v, 0.00573950773f);
+  pt = fmaf(pt, v, -0.0076224613f);
+  pt = fmaf(pt, v, 0.00943887047f);
+  pt = fmaf(pt, v, 1.00167406f);
+  pt = fmaf(pt, v, 2.83297682f);
+  return pt;
+}
+
+__device__ __forceinline__ float ndtri_giles(float u) {
+  const float w = giles_w(u);
+  const float x = giles_x(u);
+  float p = giles_central(w);
   const bool tl = w >= 5.0f;
-  if (__any(tl)) {
-    const float v = __builtin_amdgcn_sqrtf(w) - 3.0f;
-    float pt = -0.000200214257f;
-    pt = fmaf(pt, v, 0.000100950558f);
-    pt = fmaf(pt, v, 0.00134934322f);
-    pt = fmaf(pt, v, -0.00367342844f);
-    pt = fmaf(pt, v, 0.00573950773f);
-    pt = fmaf(pt, v, -0.0076224613f);
-    pt = fmaf(pt, v, 0.00943887047f);
-    pt = fmaf(pt, v, 1.00167406f);
-    pt = fmaf(pt, v, 2.83297682f);
-    p = tl ? pt : p;
-  }
+  if (__any(tl)) p = tl ? giles_tail(w) : p;
   return p * x;
 }
 

@@ -61,8 +61,11 @@ struct LinRegLds {
   const uint16_t* xl;  // [panels][npad][16] bf16 lo part (X - hi)
   const float* y;      // [npad] targets (zero on padding rows), read by the adjoint statistics only
   const float* xf;     // [panels][npad][16] FP32 column-scaled X (the plain FP32 form)
+  const double* f64;   // [3][dps] p, s and w0 in FP64 (global memory), read by the refinement form
   int npad, panels;
-  int terms;           // 1: plain FP32 MFMA; 3: BF16x3; 4: adds X hi times the third bf16 part of w - w0
+  int dps;             // stride of the per-run vectors
+  int terms;           // 1: plain FP32 MFMA; 3: BF16x3; 4: adds X hi times the third bf16 part of w - w0;
+                       // 5: the plain FP32 product with w - w0 formed in FP64 (FP64 ndtri), for refinement
 };
 
 // register slots per lane of the dispatched kernel instance for runs of at most dmax dimensions

@@ -141,12 +141,24 @@ __device__ __forceinline__ void prior_q(const float (&up)[NJ], float (&q)[NJ], u
   }
   // smallest of u and 1 - u as signed bit patterns (the order of non-negative floats, negatives below):
   // below the pattern of 1e-7f exactly when ndtri_far(u), NaN aside
+  // the central polynomial for every slot, then the tail polynomial (w >= 5, u beyond about 0.0017 of
+  // either end) behind one wavefront-uniform branch for all slots: the same values as ndtri_giles
   int m = 0x3F800000;
+  bool tail = false;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const float u = up[j];
-    q[j] = ndtri_giles(u);
+    const float w = giles_w(u);
+    q[j] = giles_central(w) * giles_x(u);
+    tail |= w >= 5.0f;
     m = min(m, min(__float_as_int(u), __float_as_int(1.0f - u)));
+  }
+  if (__any(tail)) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const float w = giles_w(up[j]);
+      if (w >= 5.0f) q[j] = giles_tail(w) * giles_x(up[j]);
+    }
   }
   if (__any(m < 0x33D6BF95)) {  // 0x33D6BF95 = 1e-7f
 #pragma unroll
@@ -307,15 +319,35 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
   const int m = l % CW;
   const int h = l / CW;
   const bool t4 = P.terms == 4;
-  const bool fp32 = P.terms == 1;
+  const bool fp32 = P.terms == 1 || P.terms == 5;
   float dwv[NJ], qv[NJ];
-  prior_q<NJ>(up, qv, tfm);
+  if (P.terms == 5) {
+    // refinement: w - w0 from the FP64 transform and FP64 parameters, rounded once to FP32 (the FP32
+    // ndtri's ~1e-7 relative error would otherwise shift w by far more than the posterior width of
+    // strongly determined weights)
+    const double* p64 = P.f64;
+    const double* s64 = P.f64 + P.dps;
+    const double* w64 = P.f64 + 2 * P.dps;
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) {
-    const int d = owned_dim<LPC>(h, j);
-    float dw = 0.0f;
-    if (j < nj && d < D) dw = fmaf(P.s[d], qv[j], P.p[d]) - P.w0[d];
-    dwv[j] = dw;
+    for (int j = 0; j < NJ; ++j) {
+      const int d = owned_dim<LPC>(h, j);
+      float dw = 0.0f;
+      if (j < nj && d < D) {
+        const double u = (double)up[j];
+        const double q = ((tfm >> j) & 1u) ? ndtri_f64(u) : u;
+        dw = (float)(fma(s64[d], q, p64[d]) - w64[d]);
+      }
+      dwv[j] = dw;
+    }
+  } else {
+    prior_q<NJ>(up, qv, tfm);
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int d = owned_dim<LPC>(h, j);
+      float dw = 0.0f;
+      if (j < nj && d < D) dw = fmaf(P.s[d], qv[j], P.p[d]) - P.w0[d];
+      dwv[j] = dw;
+    }
   }
   // publish this wavefront's B operands, negated so that the MFMA chain started from C = r0 yields
   // r = r0 - X (w - w0): BF16 forms [G][NG][3][64] vectors of 8 bf16, plain FP32 [G][NJ][64] floats
@@ -599,6 +631,9 @@ __device__ __forceinline__ void load_params(int fam, const float* prm, int dps, 
     lr->bops = (void*)(lds + 4 * dps + 4);
     lr->partial = (double*)(lds + 4 * dps + 4 + G * NG * 3 * 64 * 4);
     for (int i = tid; i < 4 * dps + 4; i += nt) lds[i] = prm[i];
+    // FP64 copies of p, s and w0 follow the FP32 block (8-byte aligned: 4 dps + 4 floats)
+    lr->f64 = (const double*)(prm + 4 * dps + 4);
+    lr->dps = dps;
     __syncthreads();
     // 1/(2 sigma^2) as the two 32-bit words of an FP64 value (little-endian)
     lr->inv2s2 = __longlong_as_double(((long long)__float_as_uint(lds[4 * dps + 1]) << 32) |
@@ -940,7 +975,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           float* dst = A.B.new_U + o * dps;
 #pragma unroll
           for (int j = 0; j < NJ; ++j)
-            if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+            dst[owned_dim<LPC>(h, j)] = U0[j];
           if (h == 0) {
             A.B.new_logL[o] = logL0;
             A.B.new_evals[o] = (int32_t)evals;
@@ -969,14 +1004,14 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
               float* pd = A.ph_U + ((int64_t)chain * kph + pidx) * dps;
 #pragma unroll
               for (int j = 0; j < NJ; ++j)
-                if (j < nj) pd[owned_dim<LPC>(h, j)] = U0[j];
+                pd[owned_dim<LPC>(h, j)] = U0[j];
               if (h == 0) A.ph_logL[(int64_t)chain * kph + pidx] = L;
             } else {
               const int64_t si = run.samp_off + phantom_base + (int64_t)chain * kph + pidx;
               float* pd = A.B.samp_U + si * dps;
 #pragma unroll
               for (int j = 0; j < NJ; ++j)
-                if (j < nj) pd[owned_dim<LPC>(h, j)] = U0[j];
+                pd[owned_dim<LPC>(h, j)] = U0[j];
               if (h == 0) {
                 A.B.samp_logL[si] = L;
                 A.B.samp_sender[si] = sender;
@@ -992,7 +1027,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
             float* dst = A.B.new_U + o * dps;
 #pragma unroll
             for (int j = 0; j < NJ; ++j)
-              if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
+              dst[owned_dim<LPC>(h, j)] = U0[j];
             if (h == 0) {
               A.B.new_logL[o] = logL0;
               A.B.new_evals[o] = (int32_t)evals;

@@ -27,8 +27,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import Buffers, Launch, RunDesc, RunState, check, lib, ptr
-from .types import (NestedSamplerResults, TerminationCondition, TerminationConditionConjunction,
-                    TerminationConditionDisjunction)
+from .types import (EvidenceCalculation, NestedSamplerResults, SampleCollection, TerminationCondition,
+                    TerminationConditionConjunction, TerminationConditionDisjunction, TerminationRegister)
 
 INT64_MAX = float(np.iinfo(np.int64).max)
 
@@ -99,7 +99,10 @@ class DeviceBatch:
 
     def __init__(self, specs: List[RunSpec], *, lanes_per_chain: int = 2, waves_per_block: int = 4,
                  max_steps: int = 512, device: str = "cuda", lr_terms: int = 3, gradient_guided: bool = False,
-                 refine_fp32: bool = False):
+                 refine_fp32: bool = False, slot_capacity: Optional[dict] = None):
+        """slot_capacity (keys N, max_samples, D, num_slices, k_phantom): every run's buffers are sized for
+        that capacity, so a finished run's slot can take another run of at most that size (run_queue);
+        the shell launches are shaped for the capacity and read each run's own sizes on the device."""
         torch = _torch()
         if not specs:
             raise ValueError("no runs")
@@ -130,15 +133,27 @@ class DeviceBatch:
         self.K = getattr(fam, "K", 1) if self.family == _lib.FAM_GMM else 1
         if self.family == _lib.FAM_GMM and any(getattr(s.model.log_likelihood, "K", 0) != self.K for s in specs):
             raise ValueError("all mixture runs of a batch must have the same number of components")
+        self.slot_capacity = dict(slot_capacity) if slot_capacity else None
+        if self.slot_capacity and (self.external or refine_fp32):
+            raise ValueError("slot capacity needs a fused likelihood family without refinement")
         maxD = max(s.model.U_ndims for s in specs)
+        if self.slot_capacity:
+            maxD = max(maxD, int(self.slot_capacity["D"]))
         if maxD > _lib.MAX_DIM:
             raise ValueError(f"U dimension {maxD} exceeds {_lib.MAX_DIM}")
         self.dps = int(math.ceil(maxD / 32) * 32)
         self.device = torch.device(device)
         dev = self.device
         dps = self.dps
-        Ns = [s.num_live_points for s in specs]
-        MSs = [s.max_samples for s in specs]
+        if self.slot_capacity:
+            cap = self.slot_capacity
+            if any(s.num_live_points > cap["N"] or s.max_samples > cap["max_samples"] for s in specs):
+                raise ValueError("a run exceeds the slot capacity")
+            Ns = [int(cap["N"])] * self.R
+            MSs = [int(cap["max_samples"])] * self.R
+        else:
+            Ns = [s.num_live_points for s in specs]
+            MSs = [s.max_samples for s in specs]
         sumN, sumMS = sum(Ns), sum(MSs)
         # parameters
         prms, consts = [], []
@@ -184,9 +199,21 @@ class DeviceBatch:
             d.param_off = int(poffs[r])
             d.seed_lo, d.seed_hi = seed_words(s.seed)
             d.log_const = consts[r]
-            lo += s.num_live_points
-            so += s.max_samples
+            lo += Ns[r]
+            so += MSs[r]
         self.runs_host = arr
+        self.param_block = [int(poffs[r + 1] - poffs[r]) for r in range(self.R)]
+        # descriptors the shell launches are shaped by: the runs themselves, or the slot capacity
+        self.runs_launch = arr
+        if self.slot_capacity:
+            cap = self.slot_capacity
+            lau = (RunDesc * self.R).from_buffer_copy(arr)
+            for r in range(self.R):
+                d = lau[r]
+                d.D, d.N, d.shell = int(cap["D"]), int(cap["N"]), int(cap["N"]) // 2
+                d.num_slices, d.k_phantom = int(cap["num_slices"]), int(cap["k_phantom"])
+                d.max_samples = int(cap["max_samples"])
+            self.runs_launch = lau
         self.runs_dev = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
         self.states_dev = torch.zeros(self.R * C.sizeof(RunState), dtype=torch.uint8, device=dev)
         f32, f64, i32 = torch.float32, torch.float64, torch.int32
@@ -228,7 +255,7 @@ class DeviceBatch:
             if self.external or self.family != _lib.FAM_LINREG or lr_terms == 1:
                 raise ValueError("refine_fp32 applies to the regression family in a BF16 form (lr_terms 3 or 4)")
             self.launch_fp32 = Launch.from_buffer_copy(self.launch)
-            self.launch_fp32.lr_terms = 1
+            self.launch_fp32.lr_terms = 5  # plain FP32 product, w - w0 from the FP64 transform
             # per run: refined accepted points whose FP32 logL is not above the shell's contour (the
             # BF16 value that decided acceptance was); they enter the live set below the contour and
             # are discarded again by the next shell
@@ -307,8 +334,9 @@ class DeviceBatch:
 
     def refine_new(self, stream=None):
         """Re-evaluate the accepted points of the current shell (every run's new samples) with the plain
-        FP32 regression likelihood and overwrite their logL (hipns_log_likelihood, one launch per run).
-        Phantom samples keep the BF16 value; evaluation counts do not include the refinement."""
+        FP32 regression product on w - w0 formed in FP64 (lr_terms 5) and overwrite their logL
+        (hipns_log_likelihood, one launch per run).  Phantom samples keep the BF16 value; evaluation
+        counts do not include the refinement."""
         s = self._s(stream)
         sz = C.sizeof(RunDesc)
         base_U, base_L, base_runs = ptr(self.new_U), ptr(self.new_logL), ptr(self.runs_dev)
@@ -344,7 +372,7 @@ class DeviceBatch:
             check(lib().hipns_shell_replace(*args, s), "hipns_shell_replace")
             check(lib().hipns_shell_register(*args, C.byref(tc), s), "hipns_shell_register")
             return
-        check(lib().hipns_collect_shell(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_host,
+        check(lib().hipns_collect_shell(ptr(self.runs_dev), ptr(self.states_dev), self.R, self.runs_launch,
                                         C.byref(self.bufs), C.byref(self.launch), C.byref(tc), None, 0,
                                         self._s(stream)), "hipns_collect_shell")
 
@@ -364,6 +392,132 @@ class DeviceBatch:
         flags = st[:, off:off + 4].contiguous().view(torch.int32).cpu().numpy().ravel()
         return flags
 
+    # ------------------------------------------------------------------ slots (run_queue)
+    def _slot_args(self, r: int):
+        sd, ss = C.sizeof(RunDesc), C.sizeof(RunState)
+        return (ptr(self.runs_dev) + r * sd, ptr(self.states_dev) + r * ss, 1, C.pointer(self.runs_host[r]),
+                C.byref(self.bufs))
+
+    def assign(self, r: int, spec: RunSpec):
+        """Give slot r (whose run has finished) the run `spec`: its descriptor and model parameters.
+        The buffers keep the slot's offsets; call init_slot next."""
+        torch = _torch()
+        if self.slot_capacity is None:
+            raise ValueError("assign needs a batch built with slot_capacity")
+        cap = self.slot_capacity
+        if spec.num_live_points > cap["N"] or spec.max_samples > cap["max_samples"] or \
+                spec.model.U_ndims > cap["D"] or spec.num_slices > cap["num_slices"] or spec.k_phantom > cap["k_phantom"]:
+            raise ValueError("run exceeds the slot capacity")
+        lik = spec.model.log_likelihood
+        if getattr(lik, "family", None) != self.family or (self.family == _lib.FAM_GMM and lik.K != self.K):
+            raise ValueError("run of another likelihood family")
+        p, c = lik.device_params(spec.model, self.dps)
+        p = p.reshape(-1) if torch.is_tensor(p) else torch.from_numpy(np.ascontiguousarray(p, dtype=np.float32))
+        if p.numel() != self.param_block[r]:
+            raise ValueError("parameter block of another size")
+        d = self.runs_host[r]
+        o = int(d.param_off)
+        self.params[o:o + p.numel()].copy_(p.to(self.device, torch.float32))
+        d.D = spec.model.U_ndims
+        d.N = spec.num_live_points
+        d.shell = spec.shell
+        d.num_slices = spec.num_slices
+        d.k_phantom = spec.k_phantom
+        d.midpoint_shrink = int(spec.midpoint_shrink)
+        d.sampler = 0
+        d.max_samples = spec.max_samples
+        d.seed_lo, d.seed_hi = seed_words(spec.seed)
+        d.log_const = c
+        sd = C.sizeof(RunDesc)
+        self.runs_dev[r * sd:(r + 1) * sd].copy_(torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8))
+        self.specs[r] = spec
+
+    def init_slot(self, r: int, term: dict, init_efficiency_threshold: float = 0.1, stream=None):
+        """create_init_state for slot r alone, then the termination checks that start a run
+        (DeviceBatch.run): the uniform phase's, which stops before its first shell for any threshold > 0,
+        then `term`'s."""
+        s = self._s(stream)
+        args = self._slot_args(r)
+        check(lib().hipns_create_init_state(*args, C.byref(self.launch), None, 0, s), "hipns_create_init_state")
+        if init_efficiency_threshold > 0:
+            ut = _lib.make_term_cond(efficiency_threshold=float(init_efficiency_threshold), dlogZ=0.0,
+                                     max_samples=float(self.runs_host[r].max_samples))
+            check(lib().hipns_check_termination(args[0], args[1], 1, args[4], C.byref(ut), 1, s),
+                  "hipns_check_termination")
+        tc = _lib.make_term_cond(**term)
+        check(lib().hipns_check_termination(args[0], args[1], 1, args[4], C.byref(tc), 1, s),
+              "hipns_check_termination")
+
+    def consume_slot(self, r: int, stream=None):
+        check(lib().hipns_consume_live(*self._slot_args(r), self._s(stream)), "hipns_consume_live")
+
+    def log_Z_slots(self, slots: List[int], stream=None) -> np.ndarray:
+        """log_Z of the runs in the given slots (their descriptors and states gathered into a compact
+        batch for hipns_batch_evidence)."""
+        torch = _torch()
+        sd, ss = C.sizeof(RunDesc), C.sizeof(RunState)
+        idx = torch.as_tensor(slots, dtype=torch.int64, device=self.device)
+        runs = self.runs_dev.view(self.R, sd).index_select(0, idx).contiguous()
+        sts = self.states_dev.view(self.R, ss).index_select(0, idx).contiguous()
+        host = (RunDesc * len(slots))(*[self.runs_host[r] for r in slots])
+        if self._ev_ws is None:
+            nbytes = int(lib().hipns_batch_evidence_workspace_size(self.R, self.runs_launch))
+            self._ev_ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        n = len(slots)
+        fins = torch.empty((n, 8), dtype=torch.float64, device=self.device)
+        phs = torch.empty(n, dtype=torch.int64, device=self.device)
+        check(lib().hipns_batch_evidence(ptr(runs), ptr(sts), n, host, C.byref(self.bufs), ptr(fins), ptr(phs),
+                                         ptr(self._ev_ws), self._ev_ws.numel(), self._s(stream)), "hipns_batch_evidence")
+        fin, ph = fins.cpu().numpy(), phs.cpu().numpy()
+        raw = sts.cpu().numpy().tobytes()
+        out = []
+        for i, r in enumerate(slots):
+            st = RunState.from_buffer_copy(raw[i * ss:(i + 1) * ss])
+            M = int(min(st.num_samples, self.runs_host[r].max_samples))
+            mu, var = _linear_to_log_stats(fin[i, 3], fin[i, 5])
+            pf = ph[i] / max(M, 1)
+            k = pf / (1.0 - pf)
+            out.append((mu, math.sqrt(var) * math.sqrt(1.0 + k)))
+        return np.array(out)
+
+    def termination_register(self, r: int) -> TerminationRegister:
+        """TerminationRegister of run r (nested_samplers/common/types.py:123-134) from its device state."""
+        st = self.states()[r]
+        return TerminationRegister(
+            num_samples_used=int(st.num_samples_used),
+            evidence_calc=EvidenceCalculation(*[float(v) for v in st.evidence_calc]),
+            evidence_calc_with_remaining=EvidenceCalculation(*[float(v) for v in st.evidence_calc_with_remaining]),
+            num_likelihood_evaluations=int(st.num_likelihood_evaluations), log_L_contour=float(st.log_L_contour),
+            efficiency=float(st.efficiency), plateau=bool(st.plateau), no_seed_points=bool(st.no_seed_points),
+            relative_spread=float(st.relative_spread), absolute_spread=float(st.absolute_spread),
+            peak_log_XL=float(st.peak_log_XL))
+
+    def sample_collection(self, r: int) -> SampleCollection:
+        """Run r's sample collection (all max_samples entries) as host arrays."""
+        d = self.runs_host[r]
+        o, ms = int(d.samp_off), int(d.max_samples)
+        return SampleCollection(sender_node_idx=self.samp_sender[o:o + ms].cpu().numpy(),
+                                log_L=self.samp_logL[o:o + ms].cpu().numpy(),
+                                U_samples=self.samp_U[o:o + ms, :int(d.D)].cpu().numpy(),
+                                num_likelihood_evaluations=self.samp_evals[o:o + ms].cpu().numpy(),
+                                phantom=self.samp_phantom[o:o + ms].cpu().numpy().astype(bool))
+
+    def report(self, r: int = 0) -> str:
+        """The per-shell report of _main_ns_thread with verbose=True (sharded_static.py:519-549)."""
+        st = self.states()[r]
+        ewr, ec = st.evidence_calc_with_remaining, st.evidence_calc
+        mu, var = _linear_to_log_stats(ewr[3], ewr[5])
+        mu0, var0 = _linear_to_log_stats(ec[3], ec[5])
+        ess = math.exp((ewr[3] + ewr[3]) - ewr[7]) if np.isfinite(ewr[7]) else float("nan")
+        return ("-------\n"
+                f"Num samples: {st.num_samples_used}\n"
+                f"Num likelihood evals: {st.num_likelihood_evaluations}\n"
+                f"Efficiency: {st.efficiency}\n"
+                f"log(L) contour: {st.log_L_contour}\n"
+                f"log(Z) est.: {mu} +- {math.sqrt(var)}\n"
+                f"log(Z | remaining) est.: {mu - mu0} +- {math.sqrt(var + var0)}\n"
+                f"ESS: {ess}\n")
+
     def cap_hits(self) -> np.ndarray:
         """Per run: slices that reached the step cap (max_steps shrinks) and kept their seed point.
         The reference's shrink loop has no cap, so a non-zero count means the run left its algorithm."""
@@ -377,10 +531,12 @@ class DeviceBatch:
 
     # ------------------------------------------------------------------ loop
     def run(self, term: dict, *, init_efficiency_threshold: float = 0.1, graph_iters: int = 0,
-            max_iterations: int = 1_000_000, stream=None):
+            max_iterations: int = 1_000_000, stream=None, verbose: bool = False):
         """create_init_state -> phases -> consume.  graph_iters > 0 replays a captured
-        hipGraph of that many shells between host checks; 0 syncs after every shell."""
+        hipGraph of that many shells between host checks; 0 syncs after every shell.  verbose prints
+        run 0's register after every host check (every shell without a graph)."""
         torch = _torch()
+        self._verbose = verbose
         self.init_state(stream)
         if init_efficiency_threshold > 0:
             uterm = dict(efficiency_threshold=float(init_efficiency_threshold), dlogZ=0.0,
@@ -403,12 +559,16 @@ class DeviceBatch:
             while not np.all(self.done_flags()) and it < max_iterations:
                 g.replay()
                 it += graph_iters
+                if getattr(self, "_verbose", False):
+                    print(self.report(0), flush=True)
         else:
             while it < max_iterations:
                 if np.all(self.done_flags()):
                     break
                 self.collect_shell(term, stream)
                 it += 1
+                if getattr(self, "_verbose", False):
+                    print(self.report(0), flush=True)
 
     def _get_graph(self, term, iters):
         torch = _torch()
@@ -603,6 +763,65 @@ class DeviceBatch:
             num_likelihood_evaluations_per_sample=evals, total_num_samples=M,
             total_phantom_samples=core["total_phantom"], total_num_likelihood_evaluations=total_evals,
             log_efficiency=float(np.log(M) - np.log(total_evals)), termination_reason=core["termination_reason"])
+
+
+def run_queue(specs: List[RunSpec], slots: int, term: dict, *, graph_iters: int = 8,
+              init_efficiency_threshold: float = 0.1, **batch_kw) -> List[dict]:
+    """Runs of one likelihood family streamed through `slots` device slots: each slot holds one run;
+    when a run terminates its live points are consumed, its log Z is formed (with the other runs that
+    finished in the same interval) and the slot takes the next run of the queue.  The slots advance
+    together through replays of a captured graph of `graph_iters` shells shaped for the slot capacity
+    (the largest num_live_points, max_samples, dimension, num_slices and k of the queue); a slot whose
+    run is done is skipped by every kernel, so runs of uneven length no longer hold the others back
+    beyond one graph interval, and the GPU stays full until the queue drains.  Each run performs the
+    same shells as in a batch of the same launch shape: its results do not depend on the slot or on
+    the other runs.  Returns, per spec in order, log_Z_mean, log_Z_uncert, termination_reason,
+    num_likelihood_evaluations, num_samples, iterations and cap_hits."""
+    if not specs:
+        return []
+    if graph_iters <= 0:
+        raise ValueError("run_queue needs graph_iters > 0")
+    cap = dict(N=max(s.num_live_points for s in specs), max_samples=max(s.max_samples for s in specs),
+               D=max(s.model.U_ndims for s in specs), num_slices=max(s.num_slices for s in specs),
+               k_phantom=max(s.k_phantom for s in specs))
+    S = min(int(slots), len(specs))
+    b = DeviceBatch(list(specs[:S]), slot_capacity=cap, **batch_kw)
+    owner = list(range(S))
+    nxt = S
+    results: List[Optional[dict]] = [None] * len(specs)
+    # the uniform phase's check on a new run's register (efficiency 0) always stops it before a shell,
+    # so every slot starts in the slice phase (init_slot)
+    for r in range(S):
+        b.init_slot(r, term, init_efficiency_threshold)
+    g = b._get_graph(term, graph_iters)
+    active = set(range(S))
+    while active:
+        flags = b.done_flags()
+        fin = sorted(r for r in active if flags[r])
+        if not fin:
+            g.replay()
+            continue
+        for r in fin:
+            b.consume_slot(r)
+        z = b.log_Z_slots(fin)
+        sts = b.states()
+        for i, r in enumerate(fin):
+            st = sts[r]
+            results[owner[r]] = dict(log_Z_mean=float(z[i, 0]), log_Z_uncert=float(z[i, 1]),
+                                     termination_reason=int(st.termination_reason),
+                                     num_likelihood_evaluations=int(st.num_likelihood_evaluations),
+                                     num_samples=int(st.num_samples), iterations=int(st.iteration),
+                                     cap_hits=int(st.cap_hits))
+        for r in fin:
+            if nxt < len(specs):
+                b.assign(r, specs[nxt])
+                owner[r] = nxt
+                nxt += 1
+                b.init_slot(r, term, init_efficiency_threshold)
+            else:
+                active.discard(r)
+    _torch().cuda.synchronize()
+    return results
 
 
 def run_concurrent(batches: List["DeviceBatch"], term, *, init_efficiency_threshold: float = 0.1,

@@ -87,7 +87,7 @@ class LinRegAdjointStats:
         Up[:, :self.D] = U
         self.U = torch.from_numpy(Up).to(dev)
         blk, npad, panels = family.data.packed(self.dps)
-        self.n_run = 4 * self.dps + 4
+        self.n_run = 4 * self.dps + 4 + 6 * self.dps  # the FP32 block, then p, s, w0 in FP64
         self.data_off = int(-(-self.n_run // 64) * 64)
         prm = np.zeros(self.data_off + blk.size, np.float32)
         prm[self.data_off:] = blk

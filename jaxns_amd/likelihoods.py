@@ -256,7 +256,8 @@ class LinearRegressionLikelihood(LikelihoodFamily):
 
     Runs of one batch share the RegressionData object (pass data=...), and may differ in sigma and
     in their priors.  Device layout per run: [q-kind: dps][p: dps][s: dps][w0: dps][1/(2 sig^2) as FP64, 0, 0]
-    in column-scaled coordinates, plus the shared block of RegressionData.packed.
+    in column-scaled coordinates, then p, s and w0 as FP64 [3][dps] (the refinement form), plus the
+    shared block of RegressionData.packed.
     """
 
     family = FAM_LINREG
@@ -298,7 +299,12 @@ class LinearRegressionLikelihood(LikelihoodFamily):
         prm[2 * dps:2 * dps + D] = np.diag(S) * c
         prm[3 * dps:3 * dps + D] = self.data.w0 * c
         prm[4 * dps:4 * dps + 2] = np.array([0.5 / self.sigma ** 2], np.float64).view(np.float32)
-        return prm, float(self.log_const)
+        # p, s and w0 again in FP64 for the refinement form (lr_terms 5)
+        f64 = np.zeros((3, dps), np.float64)
+        f64[0, :D] = loc * c
+        f64[1, :D] = np.diag(S) * c
+        f64[2, :D] = self.data.w0 * c
+        return np.concatenate([prm, f64.ravel().view(np.float32)]), float(self.log_const)
 
     def log_evidence_normal_prior(self, tau) -> float:
         """Analytic log Z for w ~ N(0, tau^2 I): log N(y; 0, sigma^2 I + tau^2 X X^T) (Woodbury form)."""

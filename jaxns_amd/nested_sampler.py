@@ -22,15 +22,33 @@ INT64_MAX = float(np.iinfo(np.int64).max)
 
 
 class NestedSamplerState:
-    """Handle on a finished run held on the device (NestedSamplerState of types.py:141-145)."""
+    """A finished run held on the device, with the fields of NestedSamplerState (types.py:141-145):
+    key, next_sample_idx, num_samples and sample_collection (copied to the host on access), plus the
+    run's TerminationRegister."""
 
     def __init__(self, batch: DeviceBatch, run_index: int = 0):
         self.batch = batch
         self.run_index = run_index
 
     @property
+    def key(self):
+        return self.batch.specs[self.run_index].seed
+
+    @property
+    def next_sample_idx(self) -> int:
+        return int(self.batch.states()[self.run_index].next_idx)
+
+    @property
     def num_samples(self) -> int:
         return int(self.batch.states()[self.run_index].num_samples)
+
+    @property
+    def sample_collection(self):
+        return self.batch.sample_collection(self.run_index)
+
+    @property
+    def termination_register(self):
+        return self.batch.termination_register(self.run_index)
 
 
 class NestedSampler:
@@ -119,7 +137,8 @@ class NestedSampler:
 
     def __call__(self, key, term_cond: Optional[TerminationCondition] = None) -> Tuple[int, NestedSamplerState]:
         batch = DeviceBatch([self._spec(key)], **self.launch_kw)
-        batch.run(self._term(term_cond), init_efficiency_threshold=self.init_efficiency_threshold)
+        batch.run(self._term(term_cond), init_efficiency_threshold=self.init_efficiency_threshold,
+                  verbose=self.verbose)
         st = batch.states()[0]
         return int(st.termination_reason), NestedSamplerState(batch, 0)
 

@@ -112,4 +112,13 @@ class TerminationRegister(NamedTuple):
     peak_log_XL: float
 
 
+class SampleCollection(NamedTuple):
+    """nested_samplers/common/types.py:115-120 (host copies of the device collection)."""
+    sender_node_idx: np.ndarray
+    log_L: np.ndarray
+    U_samples: np.ndarray
+    num_likelihood_evaluations: np.ndarray
+    phantom: np.ndarray
+
+
 TermCondType = Union[TerminationCondition, TerminationConditionConjunction, TerminationConditionDisjunction]

@@ -10,6 +10,8 @@ error.  Workloads:
             likelihood is evaluated in its Gaussian-in-w form (same function, O(D^2) per call)
   gmm       the mixture batch's first 256 run specifications (seeds 0..255: D, num_live_points,
             means and widths drawn from the seed), one oracle run per specification
+  linreg_s10, linreg_s20   the regression workload with s = 10 and 20 slices per dimension (200 runs
+            each): how the algorithm's log Z bias at s = 5 changes with longer chains
 Each oracle run uses its own stream (seed 7_000_000 + index); the device uses Philox streams, so
 the runs are statistically, not bitwise, matched.
 
@@ -36,6 +38,8 @@ WORKLOADS = {
     "mvn_d32": dict(runs=400),
     "linreg": dict(runs=400),
     "gmm": dict(runs=256),
+    "linreg_s10": dict(runs=200),
+    "linreg_s20": dict(runs=200),
 }
 
 
@@ -46,10 +50,11 @@ def _setup(name, i):
         D, N, s = (10, 300, 5) if name == "mvn_d10" else (32, 2048, 4)
         _, log_Z, mu, Sig = P.mvn_problem(D, seed=0)
         return P.mvn_oracle_spec(D, mu, Sig), N, s * D, N * 120, log_Z, 0.0, D
-    if name == "linreg":
+    if name.startswith("linreg"):
+        s = 5 if name == "linreg" else int(name.split("_s")[1])
         _, log_Z, lik = P.linreg_problem(4096, 16, seed=0, sigma=0.3, tau=1.0)
         spec, const = P.linreg_mvn_oracle_spec(lik, 1.0)
-        return spec, 1024, 16 * 5, 1024 * 200, log_Z, const, 16
+        return spec, 1024, 16 * s, 1024 * 200, log_Z, const, 16
     if name == "gmm":
         rng = np.random.default_rng(i)
         D = int(rng.integers(4, 65))

@@ -625,3 +625,77 @@ def test_uniform_sampler_phase():
         assert ok.all(), it
         assert np.all(ev >= 1) and np.all(ev <= 100)
     b.set_sampler(0)
+
+
+def test_run_queue_matches_batch():
+    """engine.run_queue streams 11 mixture runs of uneven D and num_live_points through 4 slots (a slot
+    takes the next run when its run terminates); every run's log Z, evaluation count, sample count and
+    termination reason equal those of the same runs in one batch of the same launch shape, bit for bit."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec, run_queue
+    say("run queue vs batch")
+    specs = []
+    rng = np.random.default_rng(4)
+    for i in range(11):
+        D = int(rng.integers(2, 9))
+        N = int(rng.integers(60, 200))
+        model, _, _, _ = gmm_problem(D, seed=100 + i)
+        specs.append(RunSpec(model=model, num_live_points=N, num_slices=3 * D, k_phantom=i % 2,
+                             max_samples=N * 150, seed=1000 + i))
+    term = dict(dlogZ=math.log(1.0 + 1e-3), max_samples=1e18)
+    q = run_queue(specs, 4, term, graph_iters=4, lanes_per_chain=2, waves_per_block=2)
+    cap = dict(N=max(s.num_live_points for s in specs), max_samples=max(s.max_samples for s in specs),
+               D=max(s.model.U_ndims for s in specs), num_slices=max(s.num_slices for s in specs),
+               k_phantom=max(s.k_phantom for s in specs))
+    b = DeviceBatch(specs, lanes_per_chain=2, waves_per_block=2, slot_capacity=cap)
+    b.run(term, graph_iters=4)
+    z = b.log_Z()
+    sts = b.states()
+    for i, r in enumerate(q):
+        assert r["cap_hits"] == 0
+        assert r["termination_reason"] == sts[i].termination_reason, i
+        assert r["num_likelihood_evaluations"] == sts[i].num_likelihood_evaluations, i
+        assert r["num_samples"] == sts[i].num_samples, i
+        assert (r["log_Z_mean"], r["log_Z_uncert"]) == tuple(z[i]), i
+
+
+def test_linreg_precision_at_scale():
+    """The precision-at-scale regression (65 536 rows x 128 features, 10 % of the columns x 100,
+    sigma = 0.1, w ~ N(0, I)): the device likelihood against the FP64 likelihood on the host at the same
+    (FP32) U, on prior draws and on draws from the analytic posterior (logL ~ 5.8e4).
+    - Against w = tau ndtri(U) in FP64: the FP32 prior transform's error (~1e-7 relative in q) moves the
+      strongly determined weights of the x100 columns by a good part of their posterior width, so both
+      FP32-transform forms (BF16x3 = 3, plain FP32 = 1) are within 1e-5 relative on posterior draws
+      (about 4e-6 measured), while the refinement form (5: w - w0 from the FP64 transform and FP64
+      parameters, plain FP32 product) is within 5e-7.
+    - The products alone, against the FP64 likelihood at the device's own FP32 q: plain FP32 within
+      5e-7 relative, BF16x3 with per-column scaling within 2e-5."""
+    from scipy.special import ndtri
+    torch = _torch()
+    say("linreg 65536 x 128 precision")
+    model, _, lik = linreg_problem(65536, 128, seed=0, sigma=0.1, tau=1.0, scaled_cols=0.1)
+    rng = np.random.default_rng(1)
+    B = 256
+    U = np.empty((B, 128), np.float32)
+    U[:B // 2] = rng.uniform(0.02, 0.98, size=(B // 2, 128))
+    U[B // 2:] = _linreg_posterior_U(lik, 1.0, B - B // 2, rng).clip(1e-7, 1 - 1e-7)
+    b = _batch(model, N=64, lpc=4, wpb=4)
+    assert b.dps == 128
+    Ud = torch.from_numpy(np.ascontiguousarray(U)).cuda()
+    ref64 = lik.log_prob_numpy(ndtri(U.astype(np.float64)))
+    qd = torch.empty_like(Ud)
+    check(lib().hipns_ndtri_f32(ptr(Ud), Ud.numel(), ptr(qd), _stream()))
+    ref32 = lik.log_prob_numpy(qd.cpu().numpy().astype(np.float64))
+    P, Q = slice(0, B // 2), slice(B // 2, B)
+    for terms, tol_total, tol_product in ((3, 1e-5, 2e-5), (1, 1e-5, 5e-7), (5, 5e-7, None)):
+        out = torch.empty(B, dtype=torch.float64, device="cuda")
+        b.launch.lr_terms = terms
+        check(lib().hipns_log_likelihood(ptr(Ud), B, ptr(out), ptr(b.runs_dev), b.runs_host, C.byref(b.bufs),
+                                         C.byref(b.launch), _stream()))
+        got = out.cpu().numpy()
+        tot = np.abs(got - ref64) / np.abs(ref64)
+        prod = np.abs(got - ref32) / np.abs(ref32)
+        say(f"terms {terms}: against FP64 transform prior {tot[P].max():.2e} posterior {tot[Q].max():.2e} "
+            f"(mean logL {ref64[Q].mean():.4g}); product alone prior {prod[P].max():.2e} posterior {prod[Q].max():.2e}")
+        assert tot[P].max() < 2e-5 and tot[Q].max() < tol_total
+        if tol_product is not None:
+            assert prod[Q].max() < tol_product
