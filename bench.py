@@ -39,10 +39,9 @@ PEAK_FP32_MFMA = 157.3e12
 
 # HBM bytes per k_sample launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the same
 # instrumented batch (counters scaled by a 256 MiB copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
-# profiles/k_sample_mvn_d32_n2048_r1024.md, profiles/k_sample_mvn_d32_n2048_r512.md and
-# profiles/k_sample_linreg_4096x16_n1024_r128.md.
+# profiles/k_sample_mvn_d32_n2048_r1024.md and profiles/k_sample_linreg_4096x16_n1024_r128.md.
 # Keyed by (workload, runs per launch); other shapes report null.
-MEASURED_TRAFFIC = {("c2", 512): 184.95e6, ("c2", 1024): 369.82e6, ("c4", 128): 17.68e6, ("c4", 256): 32.55e6}
+MEASURED_TRAFFIC = {("c2", 1024): 369.88e6, ("c4", 128): 17.68e6, ("c4", 256): 32.55e6}
 PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
 # wavefront VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 FP32 instruction
@@ -50,11 +49,14 @@ PEAK_HBM = 8.0e12
 PEAK_VALU_ISSUE = 256 * 4 * 2.4e9 / 2.0
 # SQ_INSTS_VALU per k_sample launch (rocprofv3 --pmc on the bench's instrumented pass of the same
 # shape), keyed like MEASURED_TRAFFIC
-MEASURED_VALU = {("c2", 512): 6.644e9, ("c2", 1024): 1.3336e10}
+MEASURED_VALU = {("c2", 1024): 1.1958e10}
 # the oracle ensemble of the same workload (tests/golden/oracle_logZ_ensembles.json) beside the
 # device's log Z statistics
-# runs of a step streamed through this many device slots (engine.run_queue) instead of one batch
-QUEUE_SLOTS = {"c3": 512}
+# runs of a step streamed through this many device slots (engine.run_queue) instead of one batch: none
+# by default (mixture batch, one step: 1024 runs in one batch 76.4 runs/s, 2048 in one batch 83.6,
+# 2048 through 1024 slots 78.4; a larger launch amortises each shell's slowest block better than slot
+# refill shortens the end of the step)
+QUEUE_SLOTS = {}
 ORACLE_ENSEMBLE_KEY = {"c1": "mvn_d10", "c2": "mvn_d32", "c4": "linreg", "c3": "gmm"}
 
 
@@ -642,7 +644,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--runs", type=int, default=None,
-                    help="runs per step (chosen form); default c2: 1024, c1: 128, c3: 1024, c4: 256, c5: 8")
+                    help="runs per step (chosen form); default c2: 1024, c1: 128, c3: 2048, c4: 256, c5: 8")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--shells", type=int, default=3, help="c5: shells per partial run")
     ap.add_argument("--full-runs", type=int, default=0, help="c5: also run this many whole runs to termination")
@@ -652,7 +654,7 @@ def main():
                          "(the roofline's instrumented launches then cover one group)")
     ap.add_argument("--slots", type=int, default=-1,
                     help="stream the runs of a step through this many device slots (engine.run_queue); "
-                         "0 = one batch; default: c3 %d, others 0" % 512)
+                         "0 = one batch (the default)")
     ap.add_argument("--lpc", type=int, default=4)
     ap.add_argument("--wpb", type=int, default=4)
     ap.add_argument("--lr-terms", type=int, default=3, choices=[3, 4])
@@ -692,7 +694,7 @@ def main():
         wl.lr_terms = args.lr_terms
     # runs per step: c2 1024 (512: 920 runs/s, 768: 939, 1024: 952 on one box; the shell kernel's tail and
     # the per-run kernels amortise over more runs); c3 1024 (256: 54.7, 512: 65.9, 1024: 74.9)
-    R = args.runs or {"c2": 1024, "c3": 1024, "c4": 256, "c5": 8}.get(args.config, 128)
+    R = args.runs or {"c2": 1024, "c3": 2048, "c4": 256, "c5": 8}.get(args.config, 128)
     G = max(1, args.groups)
     RG = (R + G - 1) // G  # runs per group: the batch one launch of the dominant kernel covers
     seed_base = 0
@@ -779,7 +781,7 @@ def main():
     log("cpu baseline", cpu)
     sec = {}
     if not args.no_secondary and args.config == "c2":
-        for cfg, Rs, st, sl in (("c1", 128, 2, 0), ("c4", 256, 1, 0), ("c3", 1024, 1, QUEUE_SLOTS["c3"])):
+        for cfg, Rs, st, sl in (("c1", 128, 2, 0), ("c4", 256, 1, 0), ("c3", 2048, 1, QUEUE_SLOTS.get("c3", 0))):
             t0 = time.perf_counter()
             sec[cfg] = secondary(cfg, Rs, st, args.graph_iters, slots=sl)
             log(f"secondary {cfg}: {sec[cfg]['runs_per_s']:.2f} runs/s in {time.perf_counter() - t0:.1f} s")

@@ -56,7 +56,9 @@ typedef struct hipns_run_desc {
   int32_t num_slices;       /* slices per chain (s * D) */
   int32_t k_phantom;        /* phantom samples kept per chain (k) */
   int32_t midpoint_shrink;  /* 1: midpoint (alpha) shrink, 0: shrink to rejected point */
-  int32_t sampler;          /* 0: UniDimSliceSampler, 1: UniformSampler (rejection) */
+  int32_t sampler;          /* 0: UniDimSliceSampler, 1: UniformSampler (rejection), 2: UniDimSliceSampler
+                             * with gradient-guided directions (uni_slice_sampler.py:255-271), d logL / dU
+                             * computed in the sampling kernel (MVN and mixture families) */
   int32_t max_uniform_evals;/* UniformSampler cap (100 in the reference) */
   int64_t max_samples;      /* sample-collection length */
   int64_t live_off;         /* offset into the live-point arrays (points) */
@@ -120,6 +122,11 @@ typedef struct hipns_buffers {
   const float* params;      /* packed model parameters */
   int32_t dps;
   int32_t pad0;
+  const int32_t* run_order; /* optional [R]: the sampling launch takes runs in this order (blockIdx.y -> run),
+                             * e.g. the longest chains first so that they do not start last; NULL = 0..R-1.
+                             * It must be a permutation of 0..R-1 for the R runs of the call it is passed
+                             * to: a call on a subset of a batch's runs (one slot, a gathered set) passes
+                             * NULL, since the batch's order would index runs outside the subset. */
 } hipns_buffers_t;
 
 /* Kernel shape picked by the autotuner: lanes per chain (2: 32 chains per
@@ -347,6 +354,15 @@ size_t hipns_mvn_prepare_workspace_size(int32_t D);
 int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const hipns_run_desc_t* runs_dev,
                          const hipns_run_desc_t* runs_host, const hipns_buffers_t* bufs, const hipns_launch_t* launch,
                          void* stream);
+
+/* The batched likelihood with its gradient d logL / dU (the gradient of Model.forward that the
+ * gradient-guided slice sampler asks for, uni_slice_sampler.py:255-258), computed as the sampling
+ * kernel computes it for sampler code 2: MVN, -(W^T r) (.) dq/du with r = W q(U) + b; mixture, the
+ * softmax-weighted component gradients times the prior scale and dq/du.  grad_dev has row stride
+ * bufs->dps (entries d < D written); FP32 products, as in the kernel.  MVN and mixture families. */
+int hipns_log_likelihood_grad(const float* U_dev, int64_t B, double* logL_dev, float* grad_dev,
+                              const hipns_run_desc_t* runs_dev, const hipns_run_desc_t* runs_host,
+                              const hipns_buffers_t* bufs, const hipns_launch_t* launch, void* stream);
 
 /* Adjoint statistics of the regression likelihood, for the M-step of evidence maximisation
  * (experimental/evidence_maximisation.py:151-160, log_evidence = logsumexp(forward(U; theta) +

@@ -16,9 +16,12 @@ from .likelihoods import (GaussianMixtureLikelihood, LinearRegressionLikelihood,
 from .nested_sampler import DefaultNestedSampler, NestedSampler, NestedSamplerState
 from .types import (NestedSamplerResults, TerminationCondition, TerminationConditionConjunction,
                     TerminationConditionDisjunction)
-from .utils import load_results, sample_evidence, save_results, summary
+from .samplers import UniDimSliceSampler, UniformSampler
+from .utils import (load_pytree, load_results, maximum_a_posteriori_point, resample, sample_evidence, save_pytree,
+                    save_results, summary)
 
 __all__ = ["Model", "Prior", "NestedSampler", "DefaultNestedSampler", "NestedSamplerState", "TerminationCondition",
            "TerminationConditionConjunction", "TerminationConditionDisjunction", "NestedSamplerResults",
            "MultivariateNormalLikelihood", "GaussianMixtureLikelihood", "LinearRegressionLikelihood", "RegressionData", "summary",
-           "save_results", "load_results", "sample_evidence", "distributions", "InvalidPriorName"]
+           "save_results", "load_results", "save_pytree", "load_pytree", "resample", "maximum_a_posteriori_point",
+           "sample_evidence", "UniDimSliceSampler", "UniformSampler", "distributions", "InvalidPriorName"]

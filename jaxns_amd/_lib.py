@@ -60,7 +60,7 @@ class Buffers(C.Structure):
     _fields_ = [("live_U", _vp * 2), ("live_logL", _vp * 2), ("live_cons", _vp * 2), ("live_sender", _vp * 2),
                 ("live_evals", _vp * 2), ("samp_U", _vp), ("samp_logL", _vp), ("samp_sender", _vp),
                 ("samp_evals", _vp), ("samp_phantom", _vp), ("new_U", _vp), ("new_logL", _vp), ("new_evals", _vp),
-                ("params", _vp), ("dps", _i32), ("pad0", _i32)]
+                ("params", _vp), ("dps", _i32), ("pad0", _i32), ("run_order", _vp)]
 
 
 class Launch(C.Structure):
@@ -125,6 +125,7 @@ def _declare(L):
     L.hipns_argsort_workspace_size.restype = C.c_size_t
     L.hipns_argsort_workspace_size.argtypes = [_i64]
     L.hipns_log_likelihood.argtypes = [_vp, _i64, _vp, _vp, P(RunDesc), P(Buffers), P(Launch), _vp]
+    L.hipns_log_likelihood_grad.argtypes = [_vp, _i64, _vp, _vp, _vp, P(RunDesc), P(Buffers), P(Launch), _vp]
     L.hipns_ext_step.argtypes = [_vp, _vp, _i32, P(RunDesc), P(Buffers), P(Launch), P(ExtState), _i32, _i32, _vp]
     L.hipns_init_prepare.argtypes = [_vp, _vp, _i32, P(RunDesc), P(Buffers), _vp]
     L.hipns_init_finalize.argtypes = [_vp, _vp, _i32, P(RunDesc), P(Buffers), _vp]
@@ -159,7 +160,7 @@ EXPORTED = ["hipns_abi_version", "hipns_last_error", "hipns_create_init_state", 
             "hipns_consume_live", "hipns_check_termination", "hipns_collect_shell_workspace_size",
             "hipns_evidence_scan", "hipns_evidence_scan_workspace_size", "hipns_count_crossed_edges",
             "hipns_count_crossed_edges_workspace_size", "hipns_argsort_stable", "hipns_argsort_workspace_size",
-            "hipns_log_likelihood", "hipns_linreg_stats", "hipns_batch_evidence",
+            "hipns_log_likelihood", "hipns_log_likelihood_grad", "hipns_linreg_stats", "hipns_batch_evidence",
             "hipns_batch_evidence_workspace_size", "hipns_ext_step", "hipns_init_prepare", "hipns_init_finalize",
             "hipns_prior_transform", "hipns_slice_chains_injected", "hipns_philox4x32",
             "hipns_ndtri_f32", "hipns_slice_quotients", "hipns_seed_indices", "hipns_slice_directions",

@@ -9,6 +9,12 @@
 #include "ns_sampler.h"
 #include "ns_shell.h"
 
+// the ABI structs have the sizes the ctypes mirrors (jaxns_amd/_lib.py) assume
+static_assert(sizeof(hipns_run_desc_t) == 88, "hipns_run_desc_t layout");
+static_assert(sizeof(hipns_run_state_t) == 264, "hipns_run_state_t layout");
+static_assert(sizeof(hipns_launch_t) == 40, "hipns_launch_t layout");
+static_assert(sizeof(hipns_buffers_t) == 168, "hipns_buffers_t layout");
+
 namespace {
 
 thread_local std::string g_err;
@@ -138,9 +144,33 @@ int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A
     A.lr_terms = lr.terms;
   }
   A.alpha_off = (int32_t)hipns::sampler_lds_floats(L->family, dps, maxD, L->gmm_K, lpc, L->waves_per_block);
-  size_t lds = ((size_t)A.alpha_off + (size_t)max_slices) * sizeof(float);
+  const size_t base_f = (size_t)A.alpha_off + (size_t)max_slices;
+  size_t lds = base_f * sizeof(float);
   if ((int)lds > max_lds_bytes())
     return fail(HIPNS_ERANGE, "model parameters need %zu B of LDS, device allows %d", lds, max_lds_bytes());
+  // direction rings after the alpha table (16-byte aligned): depth 3 for the 8-slot instances, 2 with one
+  // wavefront per block (where the per-block parameters repeat per wavefront and LDS would cap residency)
+  // and for 16 slots; none when the block's LDS would exceed the device limit
+  const int nj = hipns::sampler_nj(maxD, lpc);
+  int ring = hipns::dir_ring_max(mode, L->family, nj);
+  if (ring > 2 && L->waves_per_block < 2) ring = 2;
+  A.ring_off = (int32_t)((base_f + 3) / 4 * 4);
+  const size_t ring_lds =
+      ((size_t)A.ring_off + (size_t)ring * HIPNS_WAVE * nj * L->waves_per_block) * sizeof(float);
+  if (ring > 0 && (int)ring_lds <= max_lds_bytes()) {
+    lds = ring_lds;
+  } else {
+    ring = 0;
+  }
+  A.dir_ring = ring;
+  if (mode == hipns::MODE_GSLICE) {
+    // the gradient's scratch at the same offset: [grad_rows][64 / lpc + 1] floats per wavefront
+    lds = ((size_t)A.ring_off + (size_t)hipns::grad_rows(nj, lpc) * (HIPNS_WAVE / lpc + 1) * L->waves_per_block) *
+          sizeof(float);
+    if ((int)lds > max_lds_bytes())
+      return fail(HIPNS_ERANGE, "gradient-guided sampling needs %zu B of LDS, device allows %d", lds,
+                  max_lds_bytes());
+  }
   hipns_launch_t L2 = *L;
   L2.lanes_per_chain = lpc;
   hipError_t e;
@@ -148,6 +178,7 @@ int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A
     case hipns::MODE_SLICE: e = hipns::launch_sampler_slice(A, R, max_chains, L2, maxD, lds, s); break;
     case hipns::MODE_UNIF: e = hipns::launch_sampler_unif(A, R, max_chains, L2, maxD, lds, s); break;
     case hipns::MODE_INIT: e = hipns::launch_sampler_init(A, R, max_chains, L2, maxD, lds, s); break;
+    case hipns::MODE_GSLICE: e = hipns::launch_sampler_gslice(A, R, max_chains, L2, maxD, lds, s); break;
     default: e = hipns::launch_sampler_inj(A, R, max_chains, L2, maxD, lds, s); break;
   }
   if (e != hipSuccess) return hip_fail(e, "sampler launch");
@@ -352,8 +383,12 @@ int hipns_shell_sample(const hipns_run_desc_t* runs_dev, hipns_run_state_t* stat
   A.gmm_K = launch->gmm_K;
   int max_slices = 0;
   for (int r = 0; r < R; ++r) max_slices = runs_host[r].num_slices > max_slices ? runs_host[r].num_slices : max_slices;
-  return sampler_launch(sampler == 1 ? hipns::MODE_UNIF : hipns::MODE_SLICE, launch, A, R, maxS, maxD, bufs->dps,
-                        max_slices, as_stream(stream));
+  if (sampler == 2 && launch->family == HIPNS_FAM_LINREG)
+    return fail(HIPNS_EINVAL, "gradient-guided shells (sampler 2) run the MVN and mixture families; the regression "
+                              "family steps through hipns_ext_step");
+  if (sampler < 0 || sampler > 2) return fail(HIPNS_EINVAL, "sampler must be 0, 1 or 2");
+  const int mode = sampler == 1 ? hipns::MODE_UNIF : (sampler == 2 ? hipns::MODE_GSLICE : hipns::MODE_SLICE);
+  return sampler_launch(mode, launch, A, R, maxS, maxD, bufs->dps, max_slices, as_stream(stream));
 }
 
 int hipns_shell_replace(const hipns_run_desc_t* runs_dev, hipns_run_state_t* states_dev, int32_t R,
@@ -524,6 +559,37 @@ int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const 
   }
   hipError_t e = hipns::launch_loglik(U_dev, B, logL_dev, runs_dev, *bufs, L2, D, lds, as_stream(stream), lr);
   if (e != hipSuccess) return hip_fail(e, "loglik launch");
+  return HIPNS_OK;
+}
+
+int hipns_log_likelihood_grad(const float* U_dev, int64_t B, double* logL_dev, float* grad_dev,
+                              const hipns_run_desc_t* runs_dev, const hipns_run_desc_t* runs_host,
+                              const hipns_buffers_t* bufs, const hipns_launch_t* launch, void* stream) {
+  int rc;
+  if ((rc = check_buffers(bufs))) return rc;
+  if ((rc = check_launch(launch))) return rc;
+  if (launch->family == HIPNS_FAM_LINREG)
+    return fail(HIPNS_EINVAL, "hipns_log_likelihood_grad covers the MVN and mixture families");
+  if (!runs_host || !runs_dev) return fail(HIPNS_EINVAL, "null runs");
+  if (B < 0) return fail(HIPNS_EINVAL, "B < 0");
+  if (B == 0) return HIPNS_OK;
+  if (!U_dev || !logL_dev || !grad_dev) return fail(HIPNS_EINVAL, "null pointer");
+  const int D = runs_host[0].D;
+  if (D < 1 || D > bufs->dps) return fail(HIPNS_EINVAL, "D outside [1, dps]");
+  const int lpc = bufs->dps > 64 ? 4 : launch->lanes_per_chain;
+  const size_t base_f =
+      hipns::sampler_lds_floats(launch->family, bufs->dps, D, launch->gmm_K, lpc, launch->waves_per_block);
+  const int gscr_off = (int)((base_f + 3) / 4 * 4);
+  const int nj = hipns::sampler_nj(D, lpc);
+  const size_t lds =
+      ((size_t)gscr_off + (size_t)hipns::grad_rows(nj, lpc) * (HIPNS_WAVE / lpc + 1) * launch->waves_per_block) *
+      sizeof(float);
+  if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "parameters and gradient scratch exceed LDS");
+  hipns_launch_t L2 = *launch;
+  L2.lanes_per_chain = lpc;
+  hipError_t e =
+      hipns::launch_loglik_grad(U_dev, B, logL_dev, grad_dev, runs_dev, *bufs, L2, D, gscr_off, lds, as_stream(stream));
+  if (e != hipSuccess) return hip_fail(e, "loglik grad launch");
   return HIPNS_OK;
 }
 

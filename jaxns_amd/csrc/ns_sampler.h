@@ -21,7 +21,7 @@ struct SampleArgs {
   int32_t max_steps;
   int32_t gmm_K;
   int32_t alpha_off;  // LDS float offset of the alpha table (after the model parameters)
-  int32_t pad0;
+  int32_t dir_ring;   // slice directions drawn ahead per chain (LDS ring depth; 0 = drawn at each slice start)
   // regression family: shared data of the batch
   const float* lr_r0;
   const uint16_t* lr_xh;
@@ -30,7 +30,7 @@ struct SampleArgs {
   int32_t lr_npad;
   int32_t lr_panels;
   int32_t lr_terms;
-  int32_t pad1;
+  int32_t ring_off;   // LDS float offset of the direction rings (16-byte aligned)
   // injected-draw mode
   int32_t inj_buf;
   int32_t inj_C;
@@ -82,6 +82,13 @@ __host__ __device__ constexpr int mvn_row_stride(int nj, int lpc) {
   return ((nj * lpc + 64 / lpc - 1) / (64 / lpc)) * (64 / lpc) + 1;
 }
 
+// rows of one wavefront's gradient scratch (MVN, d logL / dU): the output rows of the MFMA blocks of
+// 64 / lpc rows that cover the nj * lpc dimension slots (32 for lpc = 2 and nj = 8, where one 32-row
+// block covers 16 slots); the scratch is [rows][64 / lpc + 1] floats
+__host__ __device__ constexpr int grad_rows(int nj, int lpc) {
+  return ((nj * lpc + 64 / lpc - 1) / (64 / lpc)) * (64 / lpc);
+}
+
 // floats of dynamic LDS one sampler block needs (layout of load_params in ns_sampler_impl.h)
 inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc, int G = 1) {
   if (fam == HIPNS_FAM_LINREG) {
@@ -98,12 +105,29 @@ inline size_t sampler_lds_floats(int fam, int dps, int D, int K, int lpc, int G 
   return (size_t)(3 + K) * dps + 2 * K;
 }
 
+// Depth of the per-chain ring of slice directions drawn ahead by an instance (0: none).  Every
+// wavefront step in which some chain of the wavefront begins a slice runs the direction draw for all
+// of its lanes, and ~43 % of chain steps begin one, so nearly every step paid for a draw.  With a ring
+// the draw runs only when a chain about to begin a slice has no direction left, and then tops up every
+// chain of the wavefront by one, so a pass serves several chains' later slices (0.96 -> ~0.6 passes per
+// step at depth 3 for 16 chains).  One ring slot is 64 lanes x nj floats per wavefront.  The regression
+// family (whose step is its data-row GEMM) and the 8-slot mixture instance (at 128 VGPRs, where the
+// ring's registers would cost a wavefront per SIMD) keep the draw at the slice start.
+__host__ __device__ constexpr int dir_ring_max(int mode, int fam, int nj) {
+  return (mode != MODE_SLICE || fam == HIPNS_FAM_LINREG || (fam == HIPNS_FAM_GMM && nj <= 8))
+             ? 0
+             : (nj <= 8 ? 3 : (nj <= 16 ? 2 : 0));
+}
+
 hipError_t launch_ext_step(const hipns_run_desc_t* runs, hipns_run_state_t* states, const hipns_buffers_t& B,
                            const hipns_ext_state_t& E, int mode, int w, int max_steps, int R, int max_chains, int lpc,
                            int wpb, int dmax, hipStream_t s);
 hipError_t launch_linreg_stats(const float* U, int64_t B, double* logL, double* rss, double* yr,
                                const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int lpc, int dmax,
                                size_t lds, hipStream_t s, const LinRegLds& lr);
+hipError_t launch_loglik_grad(const float* U, int64_t B, double* out, float* grad, const hipns_run_desc_t* runs,
+                              const hipns_buffers_t& Bf, const hipns_launch_t& L, int dmax, int gscr_off, size_t lds,
+                              hipStream_t s);
 hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
                          const hipns_buffers_t& Bf, const hipns_launch_t& L, int dmax, size_t lds, hipStream_t s,
                          const LinRegLds& lr);
@@ -119,6 +143,8 @@ hipError_t launch_sampler_init(const SampleArgs& A, int R, int max_chains, const
                                size_t lds_bytes, hipStream_t s);
 hipError_t launch_sampler_inj(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
                               size_t lds_bytes, hipStream_t s);
+hipError_t launch_sampler_gslice(const SampleArgs& A, int R, int max_chains, const hipns_launch_t& L, int dmax,
+                                 size_t lds_bytes, hipStream_t s);
 hipError_t launch_seed_indices(uint32_t k0, uint32_t k1, uint32_t it32, int first, int N, int C, int32_t* idx,
                                double* u, hipStream_t s);
 hipError_t launch_slice_dirs(uint32_t k0, uint32_t k1, uint32_t it32, int C, int num_slices, int D, int lpc,

@@ -139,28 +139,27 @@ __device__ __forceinline__ void prior_q(const float (&up)[NJ], float (&q)[NJ], u
     for (int j = 0; j < NJ; ++j) q[j] = up[j];
     return;
   }
-  // smallest of u and 1 - u as signed bit patterns (the order of non-negative floats, negatives below):
-  // below the pattern of 1e-7f exactly when ndtri_far(u), NaN aside
   // the central polynomial for every slot, then the tail polynomial (w >= 5, u beyond about 0.0017 of
-  // either end) behind one wavefront-uniform branch for all slots: the same values as ndtri_giles
-  int m = 0x3F800000;
-  bool tail = false;
+  // either end) behind one wavefront-uniform branch for all slots: the same values as ndtri_giles.  The
+  // largest w of the lane also gates the far tails: min(u, 1 - u) < 1e-7 means w > 14.7, so no lane
+  // with a slot there (ndtri_far) has w <= 14.  A u outside [0, 1] gives w = NaN, which fmaxf skips:
+  // its q is NaN and the proposal's logL -inf, as with the far-tail routine (AS 241 of u outside (0, 1)).
+  float wmax = 0.0f;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const float u = up[j];
     const float w = giles_w(u);
     q[j] = giles_central(w) * giles_x(u);
-    tail |= w >= 5.0f;
-    m = min(m, min(__float_as_int(u), __float_as_int(1.0f - u)));
+    wmax = fmaxf(wmax, w);
   }
-  if (__any(tail)) {
+  if (__any(wmax >= 5.0f)) {
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       const float w = giles_w(up[j]);
       if (w >= 5.0f) q[j] = giles_tail(w) * giles_x(up[j]);
     }
   }
-  if (__any(m < 0x33D6BF95)) {  // 0x33D6BF95 = 1e-7f
+  if (__any(wmax > 14.0f)) {
 #pragma unroll
     for (int j = 0; j < NJ; ++j)
       if (((tfm >> j) & 1u) && ndtri_far(up[j])) q[j] = ndtri_tail_f32(up[j]);
@@ -486,10 +485,22 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
   return __builtin_isnan(L) ? -__builtin_inf() : L;
 }
 
-// r = W q(u) + b ; logL = c - |r|^2 / 2 ; W rows (outputs) tiled by 32 (LPC=2) or 16 (LPC=4)
-template <int LPC, int NJ>
+// dq/du of the prior transform at a slot: 1 / phi(q) = sqrt(2 pi) exp(q^2 / 2) where q = ndtri(u), 1 for
+// the identity slots
+__device__ __forceinline__ float dq_du(float q, bool normal) {
+  return normal ? 2.50662827463100050f * __expf(0.5f * q * q) : 1.0f;
+}
+
+// r = W q(u) + b ; logL = c - |r|^2 / 2 ; W rows (outputs) tiled by 32 (LPC=2) or 16 (LPC=4).
+// GRAD: when some lane of the wavefront asks (want), also g = d logL / du = -(W^T r) (.) dq/du of the
+// owned slots.  W^T r is a second MFMA GEMM whose B operands are the lanes' own accumulators of r (the
+// K index of that product runs over r's output rows in the accumulator layout, and the A operands are
+// read from W^T at the matching rows), so r never moves; its output goes through the wavefront's LDS
+// scratch gscr ([grad_rows(NJ, LPC)][CW + 1]) into the owned-slot layout.
+template <int LPC, int NJ, bool GRAD = false>
 __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, int dps, const MvnLds& P,
-                                          double log_const, uint32_t tfm) {
+                                          double log_const, uint32_t tfm, float* gscr = nullptr,
+                                          float* g = nullptr, bool want = false) {
   constexpr int CW = HIPNS_WAVE / LPC;
   const int l = lane_id();
   const int m = l % CW;
@@ -508,6 +519,8 @@ __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, 
   const float* wt = P.WT + 8 * h * RS + m;
   float ss = 0.0f;
   const int MB = (D + CW - 1) / CW;
+  constexpr int V = (LPC == 2) ? 16 : 4;  // accumulator values per lane per output block
+  float rv[GRAD ? MBI * V : 1];
 #pragma unroll
   for (int mb = 0; mb < MBI; ++mb) {
     if (mb >= MB) break;
@@ -529,6 +542,7 @@ __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, 
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const float v = acc0[i] + acc1[i];
+        if constexpr (GRAD) rv[mb * V + i] = v;
         ss = fmaf(v, v, ss);
       }
     } else {
@@ -543,8 +557,57 @@ __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float v = acc0[i] + acc1[i];
+        if constexpr (GRAD) rv[mb * V + i] = v;
         ss = fmaf(v, v, ss);
       }
+    }
+  }
+  if constexpr (GRAD) {
+    if (__any(want)) {
+      // (W^T r)[d] for the output rows d = db * CW + (this lane's row), K over r's rows o: lane part h
+      // holds o = 16 mb + 4 h + i (LPC=4) or 32 mb + 8 (i / 4) + 4 h + i % 4 (LPC=2) in value i
+      constexpr int CWP = CW + 1;
+#pragma unroll
+      for (int db = 0; db < MBI; ++db) {
+        if (db >= MB) break;
+        const float* wd = P.WT + (db * CW + m) * RS;
+        if constexpr (LPC == 2) {
+          f32x16 acc = {0};
+#pragma unroll
+          for (int mb = 0; mb < MBI; ++mb) {
+            if (mb >= MB) break;
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+              acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wd[32 * mb + 8 * (i / 4) + 4 * h + (i % 4)], rv[mb * V + i],
+                                                        acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int i = 0; i < 16; ++i) gscr[(32 * db + 8 * (i / 4) + 4 * h + (i % 4)) * CWP + m] = acc[i];
+        } else {
+          f32x4 acc = {0};
+#pragma unroll
+          for (int mb = 0; mb < MBI; ++mb) {
+            if (mb >= MB) break;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wd[16 * mb + 4 * h + i], rv[mb * V + i], acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) gscr[(16 * db + 4 * h + i) * CWP + m] = acc[i];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int d = owned_dim<LPC>(h, j);
+        const float wr = (j < nj && d < D) ? gscr[d * CWP + m] : 0.0f;
+        g[j] = -wr * dq_du(z[j], (tfm >> j) & 1u);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
   (void)dps;
@@ -553,8 +616,12 @@ __device__ __forceinline__ double lik_mvn(const float (&up)[NJ], int nj, int D, 
   return __builtin_isnan(L) ? -__builtin_inf() : L;
 }
 
-template <int LPC, int NJ>
-__device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, const GmmLds& P, uint32_t tfm) {
+// GRAD: when some lane of the wavefront asks (want), also g = d logL / du of the owned slots:
+// d logL / dx_d = sum_k w_k (-2 inv2s2_k (x_d - m_kd)) with w_k the components' softmax weights, and
+// dx_d / du_d = q_d dq/du (x = p + q (.) q(u))
+template <int LPC, int NJ, bool GRAD = false>
+__device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, const GmmLds& P, uint32_t tfm,
+                                          float* g = nullptr, bool want = false) {
   constexpr int CW = HIPNS_WAVE / LPC;
   const int h = lane_id() / CW;
   float part[HIPNS_MAX_GMM_K];
@@ -598,6 +665,30 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
 #pragma unroll
   for (int k = 0; k < HIPNS_MAX_GMM_K; ++k)
     if (k < P.K) sum += __builtin_amdgcn_exp2f((part[k] - mx) * 1.44269504088896341f);
+  if constexpr (GRAD) {
+    if (__any(want)) {
+      float wk[HIPNS_MAX_GMM_K];
+      const float inv_sum = 1.0f / sum;
+#pragma unroll
+      for (int k = 0; k < HIPNS_MAX_GMM_K; ++k)
+        wk[k] = k < P.K ? -2.0f * P.inv2s2[k] * __builtin_amdgcn_exp2f((part[k] - mx) * 1.44269504088896341f) *
+                              inv_sum
+                        : 0.0f;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int d = owned_dim<LPC>(h, j);
+        const bool in = j < nj && d < D;
+        float dx = 0.0f;
+        if (in) {
+          const float x = fmaf(P.q[d], qv[j], P.p[d]);
+#pragma unroll
+          for (int k = 0; k < HIPNS_MAX_GMM_K; ++k)
+            if (k < P.K) dx = fmaf(wk[k], x - P.mean[k * P.dps + d], dx);
+        }
+        g[j] = in ? dx * P.q[d] * dq_du(qv[j], (tfm >> j) & 1u) : 0.0f;
+      }
+    }
+  }
   const double L = fma(-(double)ss_top, (double)i_top, (double)c_top) +
                    (double)(0.693147180559945309f * __builtin_amdgcn_logf(sum));
   return __builtin_isnan(L) ? -__builtin_inf() : L;
@@ -765,6 +856,38 @@ __device__ __forceinline__ void slice_bracket(const float (&U0)[NJ], const float
   left = chain_max<LPC>(lmax);
 }
 
+// gradient-guided direction (uni_slice_sampler.py:255-271): the Householder reflection of the last
+// direction nv about the unit gradient, n - 2 (n . g) g, renormalised; a gradient of norm below 1e-10
+// or not finite gives a new random direction (the slice's own draw)
+template <int LPC, int NJ>
+__device__ __forceinline__ void reflect_direction(float (&nv)[NJ], float (&g)[NJ], int chain, uint32_t it32, int si,
+                                                  int D, int nj, int h, uint32_t k0, uint32_t k1) {
+  float gg2 = 0.0f;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) gg2 = fmaf(g[j], g[j], gg2);
+  const float gn = __builtin_sqrtf(chain_sum<LPC>(gg2));
+  if (!(gn >= 1e-10f) || !__builtin_isfinite(gn)) {
+    slice_direction<LPC, NJ>(nv, chain, it32, si, D, nj, h, k0, k1);
+    return;
+  }
+  float dot = 0.0f;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    g[j] = g[j] / gn;
+    dot = fmaf(nv[j], g[j], dot);
+  }
+  dot = chain_sum<LPC>(dot);
+  float rn2 = 0.0f;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    nv[j] = nv[j] - 2.0f * dot * g[j];
+    rn2 = fmaf(nv[j], nv[j], rn2);
+  }
+  const float inv = 1.0f / __builtin_sqrtf(chain_sum<LPC>(rn2));
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) nv[j] = nv[j] * inv;
+}
+
 // a uniform point of the unit cube for attempt `attempt` of a chain (Model.sample_U; padding 0.5)
 template <int LPC, int NJ>
 __device__ __forceinline__ void uniform_point(float (&up)[NJ], int chain, uint32_t it32, int attempt, int D, int nj,
@@ -785,8 +908,16 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   // the regression family multiplies the data rows against every chain group of the workgroup
   // cooperatively, so its wavefronts step together (workgroup-uniform loop)
   constexpr bool COOP = (FAM == HIPNS_FAM_LINREG);
+  // gradient-guided slice chains (uni_slice_sampler.py:255-271): after each accepted slice the chain
+  // spends one step (one likelihood evaluation) on d logL / dU at the accepted point, and the next
+  // direction is the reflection of the last one about it
+  constexpr bool GG = (MODE == MODE_GSLICE);
+  constexpr bool SL = (MODE == MODE_SLICE || MODE == MODE_INJ || GG);  // slice chains
+  constexpr bool DRAWN = (MODE == MODE_SLICE || GG);                   // slice chains on Philox draws
+  static_assert(!GG || FAM != HIPNS_FAM_LINREG, "gradient-guided chains need the MVN or mixture family");
   extern __shared__ float lds[];
-  const int r = blockIdx.y;
+  // run of this block row: the batch's dispatch order (longest chains first), or the identity
+  const int r = (MODE != MODE_INJ && A.B.run_order) ? A.B.run_order[blockIdx.y] : (int)blockIdx.y;
   const hipns_run_desc_t run = A.runs[r];
   hipns_run_state_t* st = A.states + r;
   if (MODE != MODE_INJ && st->done) return;  // uniform across the block
@@ -841,7 +972,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   int cap_hits = 0;
 
   // ---- seed point (get_seed_point, uni_slice_sampler.py:343-358)
-  if (MODE == MODE_SLICE || MODE == MODE_INJ) {
+  if (SL) {
     int idx = 0;
     if (alive) {
       if (MODE == MODE_INJ) {
@@ -864,17 +995,46 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
 
   // alpha = linspace(0.5, 1, num_slices)[s] (uni_slice_sampler.py:422), one table per block
   float* alpha_tab = lds + A.alpha_off;
-  if (MODE == MODE_SLICE || MODE == MODE_INJ) {
+  if (SL) {
     for (int i = threadIdx.x; i < num_slices; i += blockDim.x)
       alpha_tab[i] = (float)(num_slices > 1 ? 0.5 + 0.5 * (double)i / (double)(num_slices - 1) : 0.5);
     __syncthreads();
   }
 
+  // ---- ring of slice directions drawn ahead (MODE_SLICE): slot b of this wavefront holds, for every
+  // lane, the nj owned components of one direction, as nj / 4 float4 rows of 64 lanes.  A chain's ring
+  // holds the directions of its slices [s, gen) (gen - s <= ring), in slot (slice index mod ring).
+  constexpr int RB = dir_ring_max(MODE, FAM, NJ);
+  constexpr int RSLOT = (NJ / 4) * HIPNS_WAVE;  // float4s of one ring slot
+  const int ring = RB > 0 ? A.dir_ring : 0;
+  float4* ring_base = nullptr;
+  if (RB > 0) ring_base = (float4*)(lds + A.ring_off) + (size_t)wave * ring * RSLOT + l;
+  int gen = 0, gslot = 0;  // next slice to draw and its slot
+  // gradient-guided: the wavefront's LDS scratch for d logL / dU, and the state of the gradient step
+  float* gscr = GG ? lds + A.ring_off + (size_t)wave * grad_rows(NJ, LPC) * (CW + 1) : nullptr;
+  bool need_grad = false, dir_ready = false;
+  float gv[NJ];
+
   // ---- begin a slice: direction and perfect bracket (the pick follows in the step)
   auto begin_slice = [&](int si) {
     k = 0;
     alpha = alpha_tab[si];
-    if (MODE == MODE_INJ) {
+    if (GG && dir_ready) {
+      dir_ready = false;  // the reflected direction of the gradient step
+    } else if (RB > 0 && ring > 0) {
+      // slice si = s is in the ring (gen > s), gen - s slots behind the next free one
+      int rslot = gslot - (gen - si);
+      if (rslot < 0) rslot += ring;
+      const float4* src = ring_base + (size_t)rslot * RSLOT;
+#pragma unroll
+      for (int j4 = 0; j4 < NJ / 4; ++j4) {
+        const float4 v = src[j4 * HIPNS_WAVE];
+        nv[4 * j4] = v.x;
+        nv[4 * j4 + 1] = v.y;
+        nv[4 * j4 + 2] = v.z;
+        nv[4 * j4 + 3] = v.w;
+      }
+    } else if (MODE == MODE_INJ) {
       const float* dsrc = A.inj_dirs + ((int64_t)chain * num_slices + si) * dps;
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
@@ -889,16 +1049,15 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
 
   auto draw_uniform_point = [&](int attempt) { uniform_point<LPC, NJ>(up, chain, it32, attempt, D, nj, h, k0, k1); };
 
-  bool need_begin = (MODE == MODE_SLICE || MODE == MODE_INJ) && alive;
+  bool need_begin = SL && alive;
   bool need_shrink = false;
 
   // the shell's phantoms are appended after its discarded points (add_phantom_samples_to_state,
   // sharded_static.py:181-207), chain-major, k per chain
   const int64_t phantom_base =
-      (MODE == MODE_SLICE && kph > 0)
-          ? append_start((st->next_idx + S) % run.max_samples, (int64_t)S * kph, run.max_samples)
-          : 0;
-  const int sender = (MODE == MODE_SLICE) ? st->sender_node : 0;
+      (DRAWN && kph > 0) ? append_start((st->next_idx + S) % run.max_samples, (int64_t)S * kph, run.max_samples)
+                         : 0;
+  const int sender = DRAWN ? st->sender_node : 0;
   const int max_uniform = (MODE == MODE_INIT) ? 1000000 : run.max_uniform_evals;
   u32x4 pickblk = {0u, 0u, 0u, 0u};
 
@@ -909,15 +1068,31 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   // word to the others.
   for (int w = 0; COOP ? __syncthreads_or(alive) : __any(alive); ++w) {
     uint32_t pickw = 0u;
-    if (MODE == MODE_SLICE) {
+    if (DRAWN) {
       if ((w & (4 * LPC - 1)) == 0)
         pickblk = philox4x32((uint32_t)chain, it32, (uint32_t)((w >> 2) + h), RNG_PICK, k0, k1);
       const int kw = w & 3;
       const uint32_t own = kw == 0 ? pickblk.x : (kw == 1 ? pickblk.y : (kw == 2 ? pickblk.z : pickblk.w));
       pickw = part_broadcast<LPC>(own, (w >> 2) & (LPC - 1));
     }
-    if (MODE == MODE_SLICE || MODE == MODE_INJ) {
-      if (alive) {
+    if (RB > 0 && ring > 0) {
+      // a chain about to begin slice s with an empty ring: one draw for every live chain whose ring has
+      // room (the same Philox counters as a draw at the slice start, so the directions are unchanged)
+      if (__any(alive && need_begin && gen == s)) {
+        if (alive && gen < num_slices && gen - s < ring) {
+          float dn[NJ];
+          slice_direction<LPC, NJ>(dn, chain, it32, gen, D, nj, h, k0, k1);
+          float4* dst = ring_base + (size_t)gslot * RSLOT;
+#pragma unroll
+          for (int j4 = 0; j4 < NJ / 4; ++j4)
+            dst[j4 * HIPNS_WAVE] = make_float4(dn[4 * j4], dn[4 * j4 + 1], dn[4 * j4 + 2], dn[4 * j4 + 3]);
+          ++gen;
+          gslot = gslot + 1 == ring ? 0 : gslot + 1;
+        }
+      }
+    }
+    if (SL) {
+      if (alive && !(GG && need_grad)) {
         if (need_begin) {
           begin_slice(s);
           need_begin = false;
@@ -937,21 +1112,27 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
         t = pick_t(left, right, u);
       }
     }
-    // ---- proposal
-    if (MODE == MODE_SLICE || MODE == MODE_INJ) {
+    // ---- proposal (a gradient step evaluates at the accepted point)
+    if (SL) {
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) up[j] = point_on_line(U0[j], t, nv[j]);
+      for (int j = 0; j < NJ; ++j) up[j] = (GG && need_grad) ? U0[j] : point_on_line(U0[j], t, nv[j]);
     } else {
       draw_uniform_point((int)evals);
     }
     // ---- batched likelihood of the wavefront's CW proposals (Model.forward)
     double L;
-    if (FAM == HIPNS_FAM_MVN)
+    if constexpr (GG) {
+      if (FAM == HIPNS_FAM_MVN)
+        L = lik_mvn<LPC, NJ, true>(up, nj, D, dps, mv, run.log_const, tfm, gscr, gv, alive && need_grad);
+      else
+        L = lik_gmm<LPC, NJ, true>(up, nj, D, gm, tfm, gv, alive && need_grad);
+    } else if (FAM == HIPNS_FAM_MVN) {
       L = lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const, tfm);
-    else if (FAM == HIPNS_FAM_GMM)
+    } else if (FAM == HIPNS_FAM_GMM) {
       L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
-    else
+    } else {
       L = lik_linreg<LPC, NJ>(up, nj, D, lr, tfm, wave, G, run.log_const);
+    }
 
     if (alive) {
       ++evals;
@@ -962,7 +1143,28 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           A.trace_logL[o] = L;
         }
       }
-      if (MODE == MODE_INIT || MODE == MODE_UNIF) {
+      if (GG && need_grad) {
+        // the gradient step (counted as one evaluation, as grad_fn is): finish after the last slice,
+        // else reflect the direction about the gradient, or draw a new one where the gradient is
+        // below 1e-10 or not finite (uni_slice_sampler.py:255-271)
+        need_grad = false;
+        if (s == num_slices) {
+          alive = false;
+          const int64_t o = run.chain_off + chain;
+          float* dst = A.B.new_U + o * dps;
+#pragma unroll
+          for (int j = 0; j < NJ; ++j)
+            dst[owned_dim<LPC>(h, j)] = U0[j];
+          if (h == 0) {
+            A.B.new_logL[o] = logL0;
+            A.B.new_evals[o] = (int32_t)evals;
+          }
+        } else {
+          reflect_direction<LPC, NJ>(nv, gv, chain, it32, s, D, nj, h, k0, k1);
+          dir_ready = true;
+          need_begin = true;
+        }
+      } else if (MODE == MODE_INIT || MODE == MODE_UNIF) {
         // uniform_sample.py:35-36 (logL > -inf) / uniform_samplers.py:44-50
         bool acc = (MODE == MODE_INIT) ? (L > -__builtin_inf()) : ((L > contour) || (L == contour));
         if (!acc && evals >= max_uniform) acc = true;
@@ -1021,7 +1223,9 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
             }
           }
           ++s;
-          if (s == num_slices) {
+          if (GG) {
+            need_grad = true;  // every accepted slice asks for the gradient, the last one included
+          } else if (s == num_slices) {
             alive = false;
             const int64_t o = run.chain_off + chain;
             float* dst = A.B.new_U + o * dps;

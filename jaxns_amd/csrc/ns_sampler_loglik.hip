@@ -44,6 +44,78 @@ __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, doubl
   if (ok && h == 0) out[row] = L;
 }
 
+// the same with d logL / dU (MVN and mixture families), the device form the gradient-guided chains
+// use: grad[row * dps + d] for d < D; the wavefront's gradient scratch follows the parameters in LDS
+template <int LPC, int NJ, int FAM>
+__global__ void __launch_bounds__(256) k_loglik_grad(const float* U, int64_t B, double* out, float* grad,
+                                                     const hipns_run_desc_t* runs, hipns_buffers_t Bf, int K,
+                                                     int gscr_off) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  extern __shared__ float lds[];
+  const hipns_run_desc_t run = runs[0];
+  const int dps = Bf.dps;
+  const int D = run.D;
+  const int DPr = ((D + 8 * LPC - 1) / (8 * LPC)) * (8 * LPC);
+  const int nj = DPr / LPC;
+  MvnLds mv;
+  GmmLds gm;
+  LinRegLds lr{};
+  const int wpb = blockDim.x / HIPNS_WAVE;
+  load_params(FAM, Bf.params + run.param_off, dps, D, K, lds, mv, gm, LPC, &lr, wpb, NJ / 8);
+  const int wave = threadIdx.x / HIPNS_WAVE;
+  const int l = lane_id();
+  const int c = l % CW, h = l / CW;
+  const int64_t row = ((int64_t)blockIdx.x * wpb + wave) * CW + c;
+  float up[NJ], g[NJ];
+  const bool ok = row < B;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    int d = owned_dim<LPC>(h, j);
+    up[j] = (ok && j < nj && d < D) ? U[row * dps + d] : 0.5f;
+  }
+  const uint32_t tfm = transform_mask<LPC, NJ>(FAM == HIPNS_FAM_MVN ? mv.tf : gm.tf, h, nj);
+  double L;
+  if (FAM == HIPNS_FAM_MVN) {
+    float* gscr = lds + gscr_off + (size_t)wave * grad_rows(NJ, LPC) * (CW + 1);
+    L = lik_mvn<LPC, NJ, true>(up, nj, D, dps, mv, run.log_const, tfm, gscr, g, true);
+  } else {
+    L = lik_gmm<LPC, NJ, true>(up, nj, D, gm, tfm, g, true);
+  }
+  if (ok) {
+    if (h == 0) out[row] = L;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int d = owned_dim<LPC>(h, j);
+      if (j < nj && d < D) grad[row * dps + d] = g[j];
+    }
+  }
+}
+
+template <int LPC, int NJ, int FAM>
+static hipError_t launch_loglik_grad_one(const float* U, int64_t B, double* out, float* grad,
+                                         const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int K, int wpb,
+                                         int gscr_off, size_t lds, hipStream_t s) {
+  constexpr int CW = HIPNS_WAVE / LPC;
+  int64_t groups = (B + CW - 1) / CW;
+  dim3 grid((unsigned)((groups + wpb - 1) / wpb)), block(wpb * HIPNS_WAVE);
+  hipLaunchKernelGGL((k_loglik_grad<LPC, NJ, FAM>), grid, block, lds, s, U, B, out, grad, runs, Bf, K, gscr_off);
+  return hipGetLastError();
+}
+
+template <int FAM>
+static hipError_t launch_loglik_grad_fam(const float* U, int64_t B, double* out, float* grad,
+                                         const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int K, int lpc,
+                                         int wpb, int dmax, int gscr_off, size_t lds, hipStream_t s) {
+  if (lpc == 4 || dmax > 64) {
+    if (dmax <= 32) return launch_loglik_grad_one<4, 8, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
+    if (dmax <= 64) return launch_loglik_grad_one<4, 16, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
+    return launch_loglik_grad_one<4, 32, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
+  }
+  if (dmax <= 16) return launch_loglik_grad_one<2, 8, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
+  if (dmax <= 32) return launch_loglik_grad_one<2, 16, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
+  return launch_loglik_grad_one<2, 32, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
+}
+
 template <int LPC, int NJ, int FAM>
 static hipError_t launch_loglik_one(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
                                    const hipns_buffers_t& Bf, int K, int wpb, size_t lds, hipStream_t s,
@@ -128,6 +200,17 @@ hipError_t launch_linreg_stats(const float* U, int64_t B, double* logL, double* 
       hipLaunchKernelGGL((k_linreg_stats<4, 32>), grid, block, lds, s, U, B, logL, rss, yr, runs, Bf, lr);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_loglik_grad(const float* U, int64_t B, double* out, float* grad, const hipns_run_desc_t* runs,
+                              const hipns_buffers_t& Bf, const hipns_launch_t& L, int dmax, int gscr_off, size_t lds,
+                              hipStream_t s) {
+  const int lpc = L.lanes_per_chain, wpb = L.waves_per_block, K = L.gmm_K;
+  if (L.family == HIPNS_FAM_GMM)
+    return launch_loglik_grad_fam<HIPNS_FAM_GMM>(U, B, out, grad, runs, Bf, K, lpc, wpb, dmax, gscr_off, lds, s);
+  if (L.family == HIPNS_FAM_MVN)
+    return launch_loglik_grad_fam<HIPNS_FAM_MVN>(U, B, out, grad, runs, Bf, K, lpc, wpb, dmax, gscr_off, lds, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_loglik(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,

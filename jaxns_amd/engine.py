@@ -111,9 +111,11 @@ class DeviceBatch:
         fam = specs[0].model.log_likelihood
         self.family = getattr(fam, "family", None)
         self.gradient_guided = bool(gradient_guided)
-        # gradient-guided slicing needs d logL / dU at accepted points, which the stepwise path
-        # takes from torch autograd; a fused family then runs through its torch form
-        self.external = self.family is None or self.gradient_guided
+        # gradient-guided slicing needs d logL / dU at accepted points: the MVN and mixture kernels compute
+        # it themselves (sampler code 2); the regression family takes it from torch autograd through the
+        # stepwise path
+        self.external = self.family is None or (self.gradient_guided and self.family == _lib.FAM_LINREG)
+        self.slice_sampler = 2 if (self.gradient_guided and not self.external) else 0
         if self.external:
             # a torch callable: evaluated between one-step sampler launches (hipns_ext_step)
             fn = fam.torch_log_prob if self.family is not None else fam
@@ -190,7 +192,7 @@ class DeviceBatch:
             d.num_slices = s.num_slices
             d.k_phantom = s.k_phantom
             d.midpoint_shrink = int(s.midpoint_shrink)
-            d.sampler = 0
+            d.sampler = self.slice_sampler
             d.max_uniform_evals = 100
             d.max_samples = s.max_samples
             d.live_off = lo
@@ -242,7 +244,18 @@ class DeviceBatch:
         b.new_U, b.new_logL, b.new_evals = ptr(self.new_U), ptr(self.new_logL), ptr(self.new_evals)
         b.params = ptr(self.params)
         b.dps = dps
+        # dispatch order of the sampling launch: runs with the longest chains (most slices) first, so
+        # that in a batch of uneven runs the slowest workgroups do not start last (stable: ties keep
+        # the run order, and a batch of equal runs keeps the identity)
+        cost = np.array([s.num_slices * max(1, s.model.U_ndims) for s in specs], np.int64)
+        order = np.argsort(-cost, kind="stable").astype(np.int32)
+        self.run_order = torch.from_numpy(order).to(dev)
+        b.run_order = ptr(self.run_order)
         self.bufs = b
+        # the same buffers for calls on a subset of the runs (one slot, a gathered set): the dispatch
+        # order indexes the whole batch, so those calls run their runs in the given order
+        self.bufs_subset = Buffers.from_buffer_copy(b)
+        self.bufs_subset.run_order = None
         if lr_terms not in (1, 3, 4):
             raise ValueError("lr_terms must be 1 (plain FP32), 3 (BF16x3) or 4")
         self.launch = Launch(self.family, lanes_per_chain, waves_per_block, self.K, lr_ndata, max_steps, lr_panels,
@@ -347,7 +360,7 @@ class DeviceBatch:
             d = self.runs_host[r]
             n, off = int(d.shell), int(d.chain_off)
             check(lib().hipns_log_likelihood(base_U + off * self.dps * 4, n, base_L + off * 8, base_runs + r * sz,
-                                             C.pointer(self.runs_host[r]), C.byref(self.bufs),
+                                             C.pointer(self.runs_host[r]), C.byref(self.bufs_subset),
                                              C.byref(self.launch_fp32), s), "hipns_log_likelihood")
             below = (self.new_logL[off:off + n] <= contour[r]).sum() * (done[r] == 0)
             self.refine_below[r:r + 1] += below
@@ -396,7 +409,7 @@ class DeviceBatch:
     def _slot_args(self, r: int):
         sd, ss = C.sizeof(RunDesc), C.sizeof(RunState)
         return (ptr(self.runs_dev) + r * sd, ptr(self.states_dev) + r * ss, 1, C.pointer(self.runs_host[r]),
-                C.byref(self.bufs))
+                C.byref(self.bufs_subset))
 
     def assign(self, r: int, spec: RunSpec):
         """Give slot r (whose run has finished) the run `spec`: its descriptor and model parameters.
@@ -424,7 +437,7 @@ class DeviceBatch:
         d.num_slices = spec.num_slices
         d.k_phantom = spec.k_phantom
         d.midpoint_shrink = int(spec.midpoint_shrink)
-        d.sampler = 0
+        d.sampler = self.slice_sampler
         d.max_samples = spec.max_samples
         d.seed_lo, d.seed_hi = seed_words(spec.seed)
         d.log_const = c
@@ -466,8 +479,9 @@ class DeviceBatch:
         n = len(slots)
         fins = torch.empty((n, 8), dtype=torch.float64, device=self.device)
         phs = torch.empty(n, dtype=torch.int64, device=self.device)
-        check(lib().hipns_batch_evidence(ptr(runs), ptr(sts), n, host, C.byref(self.bufs), ptr(fins), ptr(phs),
-                                         ptr(self._ev_ws), self._ev_ws.numel(), self._s(stream)), "hipns_batch_evidence")
+        check(lib().hipns_batch_evidence(ptr(runs), ptr(sts), n, host, C.byref(self.bufs_subset), ptr(fins),
+                                         ptr(phs), ptr(self._ev_ws), self._ev_ws.numel(), self._s(stream)),
+              "hipns_batch_evidence")
         fin, ph = fins.cpu().numpy(), phs.cpu().numpy()
         raw = sts.cpu().numpy().tobytes()
         out = []
@@ -545,7 +559,7 @@ class DeviceBatch:
             if not np.all(self.done_flags()):
                 self.set_sampler(1)
                 self._loop(uterm, graph_iters=0, max_iterations=max_iterations, stream=stream)
-                self.set_sampler(0)
+                self.set_sampler(self.slice_sampler)
         self.check_termination(term, True, stream)
         self._loop(term, graph_iters=graph_iters, max_iterations=max_iterations, stream=stream)
         self.consume(stream)
@@ -868,7 +882,7 @@ def run_concurrent(batches: List["DeviceBatch"], term, *, init_efficiency_thresh
             b = batches[i]
             with torch.cuda.stream(streams[i]):
                 if its[i] >= max_iterations or np.all(b.done_flags()):
-                    b.set_sampler(0)
+                    b.set_sampler(b.slice_sampler)
                     continue
                 b.collect_shell(uterms[i])
             its[i] += 1

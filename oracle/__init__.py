@@ -17,6 +17,7 @@ Parity status of each oracle function (how it is pinned, see tests/test_oracle.p
   count_crossed_edges -- test_tree_structure.py:19-80 KATs and the O(N^2)
       brute force count_intervals_naive on random trees.
   seed_index -- brute-force linear search over the masked set.
+  update_start / replace_index -- the reference's test_maps.py:7-30 known answers.
   ndtri -- scipy.special.ndtri.
   philox4x32 -- Random123 published known-answer vectors.
   slice chain -- invariants (bracket on the cube boundary, t inside the bracket,
@@ -113,6 +114,8 @@ def _declare(L):
     L.orc_argsort_stable.argtypes = [P(_d), _i64, P(_i64)]
     L.orc_count_crossed_edges.argtypes = [P(_i64), P(_d), _i64, _i64, P(_i64), P(_i64)]
     L.orc_count_intervals_naive.argtypes = [P(_i64), P(_d), _i64, P(_i64), P(_i64)]
+    L.orc_update_start.restype = _i64
+    L.orc_update_start.argtypes = [_i64, _i64, _i64]
     L.orc_seed_index.restype = _i64
     L.orc_seed_index.argtypes = [P(_d), _i64, _d, _d]
     L.orc_ndtri.restype = _d
@@ -234,6 +237,18 @@ def count_intervals_naive(sender, log_L):
     nl = np.empty(s.size, np.int64)
     lib().orc_count_intervals_naive(_p(s, _i64), _p(L, _d), s.size, _p(si, _i64), _p(nl, _i64))
     return si, nl
+
+
+def update_start(start, count, size):
+    return int(lib().orc_update_start(int(start), int(count), int(size)))
+
+
+def replace_index(operand, update, start):
+    """replace_index (internals/maps.py:15-25) on host arrays, through the clamped start."""
+    out = np.array(operand, copy=True)
+    j = update_start(start, len(update), len(out))
+    out[j:j + len(update)] = update
+    return out
 
 
 def seed_index(live_log_L, contour, u):

@@ -752,12 +752,23 @@ static void live_permute(live_t* dst, const live_t* src, const int64_t* perm, in
   }
 }
 
-/* _add_samples_to_state (sharded_static.py:40-85): append with wrap modulo max_samples */
+/* replace_index (internals/maps.py:15-25) writes through lax.dynamic_update_slice, which clamps the
+ * start index so that the update stays inside the operand: the first index written. */
+int64_t orc_update_start(int64_t start, int64_t count, int64_t size) {
+  int64_t last = size - count;
+  if (last < 0) last = 0;
+  if (start < 0) start = 0;
+  return start < last ? start : last;
+}
+
+/* _add_samples_to_state (sharded_static.py:40-85): write at next_sample_idx through replace_index (the
+ * clamped start), then next_sample_idx advances modulo max_samples */
 static void add_samples(int64_t* next_idx, int64_t* num_samples, int64_t max_samples, int D, int n,
                         const int64_t* sender, const double* U, const double* logL, const int64_t* evals,
                         int phantom, double* sU, double* sL, int64_t* sS, int64_t* sE, int8_t* sP) {
+  const int64_t start = orc_update_start(*next_idx, n, max_samples);
   for (int i = 0; i < n; ++i) {
-    int64_t j = (*next_idx + i) % max_samples;
+    int64_t j = start + i;
     sS[j] = sender[i];
     sL[j] = logL[i];
     sE[j] = evals ? evals[i] : 0;

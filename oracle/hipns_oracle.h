@@ -46,6 +46,9 @@ void orc_count_intervals_naive(const int64_t* sender_node_idx, const double* log
                                int64_t* samples_indices, int64_t* num_live_points);
 
 /* ---- seed selection (internals/random.py:34-84) ----------------------- */
+/* first index replace_index writes an update of `count` entries starting at `start` into `size` */
+int64_t orc_update_start(int64_t start, int64_t count, int64_t size);
+
 int64_t orc_seed_index(const double* live_log_L, int64_t N, double contour, double u);
 
 /* ---- special functions ------------------------------------------------ */

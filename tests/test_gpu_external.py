@@ -114,10 +114,21 @@ def test_external_two_priors_arguments():
 
 
 def test_gradient_guided_mvn_logZ():
-    """gradient_guided=True (uni_slice_sampler.py:255-271) with a fused-family likelihood (run through
-    its torch form): log Z against the analytic value, and every slice-sampled point spent at least
-    two evaluations per slice (its accepted proposal plus the gradient at it)."""
-    model, log_Z, _, _ = mvn_problem(5, seed=6)
+    """gradient_guided=True (uni_slice_sampler.py:255-271) in the stepwise form, the gradient from torch
+    autograd of a callable likelihood (the MVN written in torch): log Z against the analytic value, and
+    every slice-sampled point spent at least two evaluations per slice (its accepted proposal plus the
+    gradient at it)."""
+    import torch
+    fused, log_Z, mu, Sig = mvn_problem(5, seed=6)
+    P = torch.linalg.inv(torch.from_numpy(Sig))
+    mu_t = torch.from_numpy(mu)
+    c = -0.5 * 5 * math.log(2 * math.pi) - 0.5 * float(np.linalg.slogdet(Sig)[1])
+
+    def log_likelihood(x):
+        r = x - mu_t.to(x.device)
+        return c - 0.5 * ((r @ P.to(x.device)) * r).sum(-1)
+
+    model = J.Model(fused.prior_model, log_likelihood)
     ns = J.NestedSampler(model, num_live_points=150, s=3, max_samples=150 * 150, gradient_guided=True)
     reason, state = ns(9)
     res = ns.to_results(reason, state)

@@ -661,14 +661,13 @@ def test_run_queue_matches_batch():
 def test_linreg_precision_at_scale():
     """The precision-at-scale regression (65 536 rows x 128 features, 10 % of the columns x 100,
     sigma = 0.1, w ~ N(0, I)): the device likelihood against the FP64 likelihood on the host at the same
-    (FP32) U, on prior draws and on draws from the analytic posterior (logL ~ 5.8e4).
-    - Against w = tau ndtri(U) in FP64: the FP32 prior transform's error (~1e-7 relative in q) moves the
-      strongly determined weights of the x100 columns by a good part of their posterior width, so both
-      FP32-transform forms (BF16x3 = 3, plain FP32 = 1) are within 1e-5 relative on posterior draws
-      (about 4e-6 measured), while the refinement form (5: w - w0 from the FP64 transform and FP64
-      parameters, plain FP32 product) is within 5e-7.
-    - The products alone, against the FP64 likelihood at the device's own FP32 q: plain FP32 within
-      5e-7 relative, BF16x3 with per-column scaling within 2e-5."""
+    (FP32) U, on prior draws and on draws from the analytic posterior (logL ~ 5.8e4).  At the posterior
+    the weights of the x100 columns are determined to ~4e-6, so w - w0 formed in FP32 (the FP32 ndtri,
+    ~1e-7 relative in q, and the cancellation of w against w0 in column-scaled FP32 coordinates, ulp
+    ~8e-6 at 100) moves them by a good part of their posterior width: both FP32-transform forms (BF16x3 = 3,
+    plain FP32 = 1) are within 1e-5 relative on posterior draws (about 4e-6 measured; 2e-5 and 1e-6 on
+    prior draws), while the refinement form (5: w - w0 from the FP64 transform and FP64 parameters, then
+    the plain FP32 product) is within 5e-7 on both."""
     from scipy.special import ndtri
     torch = _torch()
     say("linreg 65536 x 128 precision")
@@ -686,7 +685,7 @@ def test_linreg_precision_at_scale():
     check(lib().hipns_ndtri_f32(ptr(Ud), Ud.numel(), ptr(qd), _stream()))
     ref32 = lik.log_prob_numpy(qd.cpu().numpy().astype(np.float64))
     P, Q = slice(0, B // 2), slice(B // 2, B)
-    for terms, tol_total, tol_product in ((3, 1e-5, 2e-5), (1, 1e-5, 5e-7), (5, 5e-7, None)):
+    for terms, tol_prior, tol_post in ((3, 2e-5, 1e-5), (1, 1e-6, 1e-5), (5, 5e-7, 5e-7)):
         out = torch.empty(B, dtype=torch.float64, device="cuda")
         b.launch.lr_terms = terms
         check(lib().hipns_log_likelihood(ptr(Ud), B, ptr(out), ptr(b.runs_dev), b.runs_host, C.byref(b.bufs),
@@ -695,7 +694,6 @@ def test_linreg_precision_at_scale():
         tot = np.abs(got - ref64) / np.abs(ref64)
         prod = np.abs(got - ref32) / np.abs(ref32)
         say(f"terms {terms}: against FP64 transform prior {tot[P].max():.2e} posterior {tot[Q].max():.2e} "
-            f"(mean logL {ref64[Q].mean():.4g}); product alone prior {prod[P].max():.2e} posterior {prod[Q].max():.2e}")
-        assert tot[P].max() < 2e-5 and tot[Q].max() < tol_total
-        if tol_product is not None:
-            assert prod[Q].max() < tol_product
+            f"(mean logL {ref64[Q].mean():.4g}); against the device's FP32 q prior {prod[P].max():.2e} "
+            f"posterior {prod[Q].max():.2e}")
+        assert tot[P].max() < tol_prior and tot[Q].max() < tol_post

@@ -279,7 +279,7 @@ def test_shell_discard_and_consume_placement(next_idx):
     _put_states(b, sts)
     check(lib().hipns_shell_discard(*_args(b), _lib.stream_ptr()), "hipns_shell_discard")
     g = b.states()[0]
-    start = min(next_idx, ms - S)
+    start = O.update_start(next_idx, S, ms)
     sl = slice(start, start + S)
     assert np.array_equal(b.samp_logL[sl].cpu().numpy(), live[:S])
     assert np.array_equal(b.samp_U[sl].cpu().numpy(), U[:S])
@@ -297,7 +297,7 @@ def test_shell_discard_and_consume_placement(next_idx):
     b.samp_logL.fill_(-7.0)
     check(lib().hipns_consume_live(*_args(b), _lib.stream_ptr()), "hipns_consume_live")
     g = b.states()[0]
-    start = min(next_idx, ms - N)
+    start = O.update_start(next_idx, N, ms)
     assert np.array_equal(b.samp_logL[start:start + N].cpu().numpy(), live)
     assert np.array_equal(b.samp_U[start:start + N].cpu().numpy(), U)
     assert g.num_samples == next_idx + N and g.next_idx == (next_idx + N) % ms
@@ -377,3 +377,111 @@ def test_slice_directions(D, lpc):
         p = stats.kstest(x, stats.beta(0.5, (D - 1) / 2.0).cdf).pvalue
         say("KS p", p)
         assert p > 1e-4
+
+
+# --------------------------------------------------------------------------- uniform draws (phase 1, init)
+def _uniform_U(k0, k1, it, chain, attempt, D):
+    """Model.sample_U of the fused kernel: dimension 4b + q from word q of the chain's Philox block
+    (chain, iteration, attempt, UNIF | b), as (w >> 8) 2^-24 (exact in binary32)."""
+    u = np.empty(D, np.float32)
+    for blk in range((D + 3) // 4):
+        w = O.philox4x32([chain, it, attempt, (3 << 24) | blk], [k0, k1])
+        for q in range(4):
+            if 4 * blk + q < D:
+                u[4 * blk + q] = np.float32((int(w[q]) >> 8) * 2.0 ** -24)
+    return u
+
+
+def test_uniform_draws_match_oracle():
+    """create_init_state's uniform draws (initialisation.py:20-84, uniform_sample.py:12-61) and one
+    UniformSampler shell (uniform_samplers.py:42-85) against the same draws restated on the host:
+    every initial live point is the chain's first draw (logL > -inf), and every chain of the shell
+    accepts the first draw with logL >= contour (or the 100th), with the oracle's FP64 likelihood
+    deciding; U bit for bit, evaluation counts exactly, logL to 1e-5."""
+    torch = _torch()
+    say("uniform draws vs oracle")
+    from problems import mvn_oracle_spec
+    D = 5
+    model, _, mu, Sig = mvn_problem(D, seed=1)
+    spec = mvn_oracle_spec(D, mu, Sig)
+    b = _batch((64,), D=D)
+    k0, k1 = int(b.runs_host[0].seed_lo), int(b.runs_host[0].seed_hi)
+    b.init_state()
+    st = b.states()[0]
+    N, S = 64, int(b.runs_host[0].shell)
+    live_U = b.live_U[st.buf][:N, :D].cpu().numpy()
+    host = np.stack([_uniform_U(k0, k1, 0xFFFFFFFF, c, 0, D) for c in range(N)])
+    key = lambda a: sorted(map(bytes, np.ascontiguousarray(a)))  # noqa: E731
+    assert key(live_U) == key(host)
+    assert np.all(b.live_evals[st.buf][:N].cpu().numpy() == 1)
+    # one uniform-sampler shell from the initial live set
+    b.set_sampler(1)
+    args = _args(b)
+    check(lib().hipns_shell_discard(*args, _lib.stream_ptr()), "hipns_shell_discard")
+    check(lib().hipns_shell_sample(*args, C.byref(b.launch), _lib.stream_ptr()), "hipns_shell_sample")
+    st = b.states()[0]
+    contour, it = st.contour, int(st.iteration)
+    U = b.new_U[:S, :D].cpu().numpy()
+    L = b.new_logL[:S].cpu().numpy()
+    ev = b.new_evals[:S].cpu().numpy()
+    checked = 0
+    for c in range(S):
+        a, close = 0, False
+        while True:
+            u = _uniform_U(k0, k1, it, c, a, D)
+            lo = float(spec.log_likelihood(u.astype(np.float64))[0])
+            close |= abs(lo - contour) < 1e-4 * max(1.0, abs(contour))
+            a += 1
+            if lo >= contour or a >= 100:
+                break
+        if close:
+            continue  # an FP32-vs-FP64 difference could flip this chain's decision
+        checked += 1
+        assert ev[c] == a, c
+        assert np.array_equal(U[c], u), c
+        assert abs(L[c] - lo) <= 1e-5 * max(1.0, abs(lo)), c
+    say("chains checked", checked, "of", S)
+    assert checked >= S - 2
+    b.set_sampler(0)
+
+
+def test_sampler_plugin_reproduces_a_shell():
+    """The sampler plugin interface (samplers/abc.py:18-115; UniDimSliceSampler,
+    uni_slice_sampler.py:276-441): pre_process of the live points, then get_samples with a run's seed,
+    shell index and contour, gives that shell's new samples bit for bit (the engine's shell and the
+    plugin run the same kernel on the same stream); every sample is above the contour, phantom samples
+    count k per sample with no likelihood evaluations, and get_sample returns one sample."""
+    import torch
+    from jaxns_amd.samplers import EphemeralState, LivePointCollection, UniDimSliceSampler
+    say("sampler plugin vs shell")
+    D, N, k = 6, 128, 2
+    model, _, _, _ = mvn_problem(D, seed=3)
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    spec = RunSpec(model=model, num_live_points=N, num_slices=3 * D, k_phantom=k, max_samples=N * 40, seed=77)
+    b = DeviceBatch([spec], lanes_per_chain=2, waves_per_block=2)
+    b.init_state()
+    for _ in range(3):
+        b.collect_shell(dict(max_samples=1e18))
+    st = b.states()[0]
+    cur = st.buf
+    U = b.live_U[cur][:N, :D].double().cpu().numpy()
+    L = b.live_logL[cur][:N].cpu().numpy()
+    S = int(b.runs_host[0].shell)
+    args = _args(b)
+    check(lib().hipns_shell_discard(*args, _lib.stream_ptr()), "hipns_shell_discard")
+    check(lib().hipns_shell_sample(*args, C.byref(b.launch), _lib.stream_ptr()), "hipns_shell_sample")
+    st = b.states()[0]
+    want_U = b.new_U[:S, :D].double().cpu().numpy()
+    want_L = b.new_logL[:S].cpu().numpy()
+    sampler = UniDimSliceSampler(model, num_slices=3 * D, num_phantom_save=k, midpoint_shrink=True)
+    assert sampler.num_phantom() == k
+    lp = LivePointCollection(sender_node_idx=np.zeros(N, np.int64), U_sample=U, log_L_constraint=np.zeros(N),
+                             log_L=L, num_likelihood_evaluations=np.ones(N, np.int64))
+    state = sampler.pre_process(EphemeralState(key=0, live_points_collection=lp))
+    s, ph = sampler.get_samples((77, int(st.iteration)), st.contour, state, S)
+    assert np.array_equal(s.U_sample, want_U)
+    assert np.array_equal(s.log_L, want_L)
+    assert np.all(s.log_L > st.contour)
+    assert ph.log_L.shape == (S * k,) and np.all(ph.num_likelihood_evaluations == 0) and np.all(ph.log_L > st.contour)
+    one, _ = sampler.get_sample(5, st.contour, state)
+    assert one.U_sample.shape == (D,) and one.log_L > st.contour

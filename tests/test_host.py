@@ -42,6 +42,7 @@ def test_struct_layouts():
     assert C.sizeof(_lib.RunState) == 264
     assert C.sizeof(_lib.TermCond) == 10 * 4 + 10 * 8
     assert C.sizeof(_lib.Launch) == 40
+    assert C.sizeof(_lib.Buffers) == 168
 
 
 def test_round_up_kats():

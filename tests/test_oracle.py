@@ -377,3 +377,15 @@ def test_plateau_acceptance_rule():
     rep = np.array([[0.5, 1.0, 0, 0, 0, 0, 0, 0]])
     o = O.slice_chain(spec, np.array([0.5]), 1.0, 1.0, 1, np.ones((1, 1)), tu, logL_replay=rep)
     assert o["steps"][0] == 2 and o["logL"] == 1.0
+
+
+# ---------------------------------------------------------------- sample-collection append
+def test_replace_index_kats():
+    """replace_index (internals/maps.py:15-25) known answers of internals/tests/test_maps.py:7-30: an
+    update that would run past the end is shifted back (lax.dynamic_update_slice clamps the start)."""
+    op = np.array([0, 1, 2, 3, 4])
+    assert list(O.replace_index(op, [5, 5], 0)) == [5, 5, 2, 3, 4]
+    assert list(O.replace_index(op, [5], 0)) == [5, 1, 2, 3, 4]
+    assert list(O.replace_index(op, [5, 5], 4)) == [0, 1, 2, 5, 5]
+    assert list(O.replace_index(op, [5], 4)) == [0, 1, 2, 3, 5]
+    assert O.update_start(150, 32, 160) == 128 and O.update_start(100, 32, 160) == 100
