@@ -143,7 +143,8 @@ typedef struct hipns_launch {
   int32_t lr_panels;        /* LinReg: 16-column panels of X (even, 16 * lr_panels >= D rounded up to 32) */
   int32_t lr_terms;         /* LinReg products: 3 (or 0) = BF16x3 hi.hi + hi.lo + lo.hi; 4 adds X hi times the third
                              * bf16 part of w - w0; 1 = plain FP32 (FP32 X and w, FP32 MFMA); 5 = the plain FP32
-                             * product with w - w0 formed in FP64 (FP64 ndtri and parameters; refinement) */
+                             * product with w - w0 formed in FP64 (FP64 ndtri and parameters; refinement),
+                             * hipns_log_likelihood only (sampling launches and hipns_linreg_stats take 1, 3, 4) */
   int64_t lr_data_off;      /* LinReg: float offset in params of [r0 | X hi | X lo | y | X] shared by all runs;
                              * X parts are [lr_panels][lr_ndata][16], bf16 hi/lo then f32 */
 } hipns_launch_t;

@@ -132,6 +132,8 @@ int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A
   int lpc = (dps > 64) ? 4 : L->lanes_per_chain;
   hipns::SampleArgs A = A0;
   if (L->family == HIPNS_FAM_LINREG) {
+    if (L->lr_terms == 5)
+      return fail(HIPNS_EINVAL, "lr_terms 5 (the refinement form) is for hipns_log_likelihood; sampling takes 1, 3 or 4");
     int rc = check_panels(L, maxD, lpc);
     if (rc) return rc;
     const hipns::LinRegLds lr = linreg_view(&A.B, L);
@@ -600,6 +602,7 @@ int hipns_linreg_stats(const float* U_dev, int64_t B, double* logL_dev, double* 
   if ((rc = check_buffers(bufs))) return rc;
   if ((rc = check_launch(launch))) return rc;
   if (launch->family != HIPNS_FAM_LINREG) return fail(HIPNS_EINVAL, "hipns_linreg_stats needs the regression family");
+  if (launch->lr_terms == 5) return fail(HIPNS_EINVAL, "hipns_linreg_stats takes lr_terms 1, 3 or 4");
   if (!runs_host || !runs_dev) return fail(HIPNS_EINVAL, "null runs");
   if (B < 0) return fail(HIPNS_EINVAL, "B < 0");
   if (B == 0) return HIPNS_OK;

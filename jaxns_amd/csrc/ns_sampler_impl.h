@@ -308,7 +308,10 @@ __device__ __forceinline__ void lr_bf16_tiles(const LinRegLds& P, const bf16x8_t
   }
 }
 
-template <int LPC, int NJ, bool STATS = false>
+// F64T: the instance also carries the refinement form (terms 5: the FP64 transform).  Only the batched
+// likelihood kernel needs it; in the sampling kernel its FP64 registers would cost a wavefront per SIMD
+// (256 VGPRs against 214 at 8 slots and LPC=2), so there terms 5 is rejected by the host.
+template <int LPC, int NJ, bool STATS = false, bool F64T = false>
 __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int D, const LinRegLds& P, uint32_t tfm,
                                              int g, int G, double log_const, double* rss = nullptr,
                                              double* yr = nullptr) {
@@ -318,9 +321,9 @@ __device__ __forceinline__ double lik_linreg(const float (&up)[NJ], int nj, int 
   const int m = l % CW;
   const int h = l / CW;
   const bool t4 = P.terms == 4;
-  const bool fp32 = P.terms == 1 || P.terms == 5;
+  const bool fp32 = P.terms == 1 || (F64T && P.terms == 5);
   float dwv[NJ], qv[NJ];
-  if (P.terms == 5) {
+  if (F64T && P.terms == 5) {
     // refinement: w - w0 from the FP64 transform and FP64 parameters, rounded once to FP32 (the FP32
     // ndtri's ~1e-7 relative error would otherwise shift w by far more than the posterior width of
     // strongly determined weights)

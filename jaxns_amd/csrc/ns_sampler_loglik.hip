@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, doubl
   else if (FAM == HIPNS_FAM_GMM)
     L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
   else
-    L = lik_linreg<LPC, NJ>(up, nj, D, lr, tfm, wave, wpb, run.log_const);
+    L = lik_linreg<LPC, NJ, false, true>(up, nj, D, lr, tfm, wave, wpb, run.log_const);
   if (ok && h == 0) out[row] = L;
 }
 
