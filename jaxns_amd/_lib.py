@@ -89,14 +89,20 @@ def build(force: bool = False, jobs: int = 8) -> str:
     return LIB_PATH
 
 
+ABI_VERSION = 1  # HIPNS_ABI_VERSION of include/hipns.h that these struct mirrors follow
+
+
 def lib():
-    """Load libhipns.so; raises HipnsError when it has not been built."""
+    """Load libhipns.so; raises HipnsError when it has not been built or is of another ABI version."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HipnsError(f"{LIB_PATH} is missing: build it with `make -C {CSRC}` (or __graft_entry__.build())")
         L = C.CDLL(LIB_PATH)
         _declare(L)
+        if L.hipns_abi_version() != ABI_VERSION:
+            raise HipnsError(f"{LIB_PATH} has ABI version {L.hipns_abi_version()}, these bindings need "
+                             f"{ABI_VERSION}: rebuild it with `make -C {CSRC}`")
         _lib = L
     return _lib
 

@@ -44,10 +44,12 @@ def _device_ensemble(name, R, seed0, groups=1):
                  for i in range(R)]
         truths = [log_Z] * R
         lpc = 4
-    elif name == "linreg":
+    elif name.startswith("linreg"):
+        # s = 5 slices per dimension (the benchmark workload), or 10 and 20 (the oracle's s-sweep)
+        s = {"linreg": 5, "linreg_s10": 10, "linreg_s20": 20}[name]
         model, log_Z, _ = linreg_problem(4096, 16, seed=0, sigma=0.3, tau=1.0)
-        specs = [RunSpec(model=model, num_live_points=1024, num_slices=80, max_samples=1024 * 200, seed=seed0 + i)
-                 for i in range(R)]
+        specs = [RunSpec(model=model, num_live_points=1024, num_slices=16 * s, max_samples=1024 * 200,
+                         seed=seed0 + i) for i in range(R)]
         truths = [log_Z] * R
         lpc = 2
     else:
@@ -73,7 +75,8 @@ def _device_ensemble(name, R, seed0, groups=1):
     return z[:, 0] - np.array(truths), z[:, 1], caps
 
 
-@pytest.mark.parametrize("name,R", [("mvn_d10", 400), ("mvn_d32", 512), ("linreg", 256), ("gmm", 256)])
+@pytest.mark.parametrize("name,R", [("mvn_d10", 400), ("mvn_d32", 512), ("linreg", 256), ("linreg_s10", 256),
+                                    ("linreg_s20", 256), ("gmm", 256)])
 def test_device_ensemble_matches_oracle(name, R):
     fx = _fixture(name)["summary"]
     say(f"ensemble {name}: {R} device runs against {fx['runs']} oracle runs")
