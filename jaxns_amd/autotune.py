@@ -2,7 +2,8 @@
 
 Candidates are (lanes_per_chain, waves_per_block): 2 lanes per chain = 32
 chains per wavefront with 32x32 MFMA tiles, 4 lanes per chain = 16 chains per
-wavefront with 16x16 tiles.  For the dense-Gaussian and mixture families the
+wavefront with 16x16 tiles; the mixture family (no MFMA) also tries 8 lanes per
+chain, 8 chains per wavefront with half the register slots of 4 lanes.  For the dense-Gaussian and mixture families the
 workgroup holds 1-4 independent wavefronts of chains; for the regression family
 its 1-4 wavefronts share one group of chains and split the data rows.  Each
 candidate runs a few shells of a trial batch of the real shape (the same run
@@ -21,6 +22,7 @@ from typing import Iterable, List, Optional, Tuple
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), ".autotune_cache.json")
 DEFAULT_CANDIDATES = ((2, 1), (2, 2), (2, 4), (4, 1), (4, 2), (4, 4))
+MIXTURE_CANDIDATES = DEFAULT_CANDIDATES + ((8, 1), (8, 2), (8, 4))
 
 
 def _key(specs, lr_terms) -> str:
@@ -33,7 +35,7 @@ def _key(specs, lr_terms) -> str:
     if fam == 2:
         extra = f"-rows{lik.N}-t{lr_terms}"
     h = hashlib.sha1(repr(shape).encode()).hexdigest()[:12]
-    return f"fam{fam}{extra}-R{len(specs)}-{h}-{name}"
+    return f"fam{fam}{extra}-R{len(specs)}-{h}-{name}-c2"
 
 
 def measure(specs, lpc, wpb, shells=3, lr_terms=3) -> float:
@@ -82,7 +84,11 @@ def autotune(specs: List, candidates: Optional[Iterable[Tuple[int, int]]] = None
     if key in cache:
         return tuple(cache[key]["best"])
     scores = {}
-    for lpc, wpb in (candidates or DEFAULT_CANDIDATES):
+    if candidates is None:
+        from ._lib import FAM_GMM
+        mix = specs[0].model.log_likelihood.family == FAM_GMM and max(s.model.U_ndims for s in specs) <= 128
+        candidates = MIXTURE_CANDIDATES if mix else DEFAULT_CANDIDATES
+    for lpc, wpb in candidates:
         scores[(lpc, wpb)] = measure(specs, lpc, wpb, shells=shells, lr_terms=lr_terms)
         if verbose:
             print(f"[autotune] lpc={lpc} wpb={wpb}: {scores[(lpc, wpb)] / 1e9:.3f} G evals/s", flush=True)

@@ -85,8 +85,8 @@ int check_launch(const hipns_launch_t* L) {
     if (L->lr_terms != 0 && L->lr_terms != 1 && L->lr_terms != 3 && L->lr_terms != 4 && L->lr_terms != 5)
       return fail(HIPNS_EINVAL, "lr_terms must be 1, 3, 4 or 5");
   }
-  if (L->lanes_per_chain != 2 && L->lanes_per_chain != 4)
-    return fail(HIPNS_EINVAL, "lanes_per_chain must be 2 or 4");
+  if (L->lanes_per_chain != 2 && L->lanes_per_chain != 4 && !(L->lanes_per_chain == 8 && L->family == HIPNS_FAM_GMM))
+    return fail(HIPNS_EINVAL, "lanes_per_chain must be 2 or 4 (or 8 for the mixture family)");
   if (L->waves_per_block < 1 || L->waves_per_block > 4) return fail(HIPNS_EINVAL, "waves_per_block must be 1..4");
   if (L->family == HIPNS_FAM_GMM && (L->gmm_K < 1 || L->gmm_K > HIPNS_MAX_GMM_K))
     return fail(HIPNS_EINVAL, "gmm_K must be 1..%d", HIPNS_MAX_GMM_K);
@@ -129,7 +129,7 @@ int max_lds_bytes() {
 
 int sampler_launch(int mode, const hipns_launch_t* L, const hipns::SampleArgs& A0, int R, int max_chains, int maxD,
                    int dps, int max_slices, hipStream_t s) {
-  int lpc = (dps > 64) ? 4 : L->lanes_per_chain;
+  int lpc = hipns::sampler_lpc(L->family, L->lanes_per_chain, dps, maxD);
   hipns::SampleArgs A = A0;
   if (L->family == HIPNS_FAM_LINREG) {
     if (L->lr_terms == 5)
@@ -548,7 +548,7 @@ int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const 
   if (!U_dev || !logL_dev) return fail(HIPNS_EINVAL, "null pointer");
   const int D = runs_host[0].D;
   if (D < 1 || D > bufs->dps) return fail(HIPNS_EINVAL, "D outside [1, dps]");
-  int lpc = bufs->dps > 64 ? 4 : launch->lanes_per_chain;
+  int lpc = hipns::sampler_lpc(launch->family, launch->lanes_per_chain, bufs->dps, D);
   size_t lds = hipns::sampler_lds_floats(launch->family, bufs->dps, D, launch->gmm_K, lpc, launch->waves_per_block) *
                sizeof(float);
   if ((int)lds > max_lds_bytes()) return fail(HIPNS_ERANGE, "parameters exceed LDS");
@@ -578,7 +578,7 @@ int hipns_log_likelihood_grad(const float* U_dev, int64_t B, double* logL_dev, f
   if (!U_dev || !logL_dev || !grad_dev) return fail(HIPNS_EINVAL, "null pointer");
   const int D = runs_host[0].D;
   if (D < 1 || D > bufs->dps) return fail(HIPNS_EINVAL, "D outside [1, dps]");
-  const int lpc = bufs->dps > 64 ? 4 : launch->lanes_per_chain;
+  const int lpc = hipns::sampler_lpc(launch->family, launch->lanes_per_chain, bufs->dps, D);
   const size_t base_f =
       hipns::sampler_lds_floats(launch->family, bufs->dps, D, launch->gmm_K, lpc, launch->waves_per_block);
   const int gscr_off = (int)((base_f + 3) / 4 * 4);

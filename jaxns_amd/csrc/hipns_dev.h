@@ -40,12 +40,27 @@ __device__ __forceinline__ void swap32(float v, float& a, float& b) {
   b = __uint_as_float(r[1]);
 }
 
+// the value of lane l ^ 8 (LPC = 8: the partner 8 lanes away, inside the same 16-lane row): a DPP
+// row rotation by 8
+__device__ __forceinline__ uint32_t xor8(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
+}
+__device__ __forceinline__ float xor8f(float v) { return __uint_as_float(xor8(__float_as_uint(v))); }
+
 // value of lane part `part` (wavefront-uniform) of this lane's chain, on every part of the chain:
 // r[0] / r[1] of a swap are the values of the lower / upper partner
 template <int LPC>
 __device__ __forceinline__ uint32_t part_broadcast(uint32_t v, int part) {
   auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-  if constexpr (LPC == 4) {
+  if constexpr (LPC == 8) {
+    v = (part & 4) ? r[1] : r[0];
+    auto q = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    v = (part & 2) ? q[1] : q[0];
+    // bit 0 of the part is bit 3 of the lane: keep this lane's value where they agree, else the
+    // partner's
+    const uint32_t x = xor8(v);
+    return (((lane_id() >> 3) & 1) == (part & 1)) ? v : x;
+  } else if constexpr (LPC == 4) {
     v = (part & 2) ? r[1] : r[0];
     auto q = __builtin_amdgcn_permlane16_swap(v, v, false, false);
     return (part & 1) ? q[1] : q[0];
@@ -53,10 +68,13 @@ __device__ __forceinline__ uint32_t part_broadcast(uint32_t v, int part) {
   return part ? r[1] : r[0];
 }
 
+// reductions over the LPC lanes of a chain; with a commutative operation the partners compute the
+// same value (a + b = b + a in IEEE arithmetic)
 template <int LPC>
 __device__ __forceinline__ float chain_sum(float v) {
   float a, b;
-  if constexpr (LPC == 4) {
+  if constexpr (LPC == 8) v = v + xor8f(v);
+  if constexpr (LPC >= 4) {
     swap16(v, a, b);
     v = a + b;
   }
@@ -66,7 +84,8 @@ __device__ __forceinline__ float chain_sum(float v) {
 template <int LPC>
 __device__ __forceinline__ float chain_min(float v) {
   float a, b;
-  if constexpr (LPC == 4) {
+  if constexpr (LPC == 8) v = fminf(v, xor8f(v));
+  if constexpr (LPC >= 4) {
     swap16(v, a, b);
     v = fminf(a, b);
   }
@@ -76,7 +95,8 @@ __device__ __forceinline__ float chain_min(float v) {
 template <int LPC>
 __device__ __forceinline__ float chain_max(float v) {
   float a, b;
-  if constexpr (LPC == 4) {
+  if constexpr (LPC == 8) v = fmaxf(v, xor8f(v));
+  if constexpr (LPC >= 4) {
     swap16(v, a, b);
     v = fmaxf(a, b);
   }

@@ -71,8 +71,19 @@ struct LinRegLds {
 // register slots per lane of the dispatched kernel instance for runs of at most dmax dimensions
 // (the launchers select the same instance)
 __host__ __device__ inline int sampler_nj(int dmax, int lpc) {
+  if (lpc == 8) return dmax <= 64 ? 8 : 16;  // the mixture family, dmax <= 128
   if (lpc == 2 && dmax <= 64) return dmax <= 16 ? 8 : (dmax <= 32 ? 16 : 32);
   return dmax <= 32 ? 8 : (dmax <= 64 ? 16 : 32);
+}
+
+// lanes per chain of a launch: 8 only for the mixture family (no MFMA tile ties its layout), and only
+// where the U row stride covers every dimension slot of the 8-lane instance (8 slots: 64 dimensions,
+// 16 slots: 128), since the kernels read and write all owned slots of a row; else 4 above 64
+// dimensions (the widest instance), else the requested 2 or 4
+__host__ __device__ inline int sampler_lpc(int family, int lanes_per_chain, int dps, int dmax) {
+  if (lanes_per_chain == 8)
+    return (family == HIPNS_FAM_GMM && dmax <= 128 && dps >= 8 * sampler_nj(dmax, 8)) ? 8 : 4;
+  return dps > 64 ? 4 : lanes_per_chain;
 }
 
 // MVN: row stride of the transposed whitening matrix in LDS for the instance (NJ, LPC): the instance's

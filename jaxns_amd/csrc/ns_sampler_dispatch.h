@@ -25,7 +25,15 @@ inline hipError_t launch_one(const SampleArgs& A, int R, int max_chains, int wpb
 template <int FAM, int MODE>
 inline hipError_t launch_lpc(const SampleArgs& A, int R, int max_chains, int lpc, int wpb, int dmax, size_t lds,
                              hipStream_t s) {
-  if (lpc == 4 || dmax > 64) {
+  if constexpr (FAM == HIPNS_FAM_GMM) {
+    // the mixture family also runs 8 lanes (8 slots) per chain: 8 chains per wavefront, up to 64 dims
+    // in 8 slots, and about half the registers of the 16-slot instance of 4 lanes per chain
+    if (lpc == 8 && dmax <= 128) {
+      if (dmax <= 64) return launch_one<8, 8, FAM, MODE>(A, R, max_chains, wpb, lds, s);
+      return launch_one<8, 16, FAM, MODE>(A, R, max_chains, wpb, lds, s);
+    }
+  }
+  if (lpc == 4 || lpc == 8 || dmax > 64) {
     if (dmax <= 32) return launch_one<4, 8, FAM, MODE>(A, R, max_chains, wpb, lds, s);
     if (dmax <= 64) return launch_one<4, 16, FAM, MODE>(A, R, max_chains, wpb, lds, s);
     return launch_one<4, 32, FAM, MODE>(A, R, max_chains, wpb, lds, s);

@@ -1125,13 +1125,13 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
     // ---- batched likelihood of the wavefront's CW proposals (Model.forward)
     double L;
     if constexpr (GG) {
-      if (FAM == HIPNS_FAM_MVN)
+      if constexpr (FAM == HIPNS_FAM_MVN)
         L = lik_mvn<LPC, NJ, true>(up, nj, D, dps, mv, run.log_const, tfm, gscr, gv, alive && need_grad);
       else
         L = lik_gmm<LPC, NJ, true>(up, nj, D, gm, tfm, gv, alive && need_grad);
-    } else if (FAM == HIPNS_FAM_MVN) {
+    } else if constexpr (FAM == HIPNS_FAM_MVN) {
       L = lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const, tfm);
-    } else if (FAM == HIPNS_FAM_GMM) {
+    } else if constexpr (FAM == HIPNS_FAM_GMM) {
       L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
     } else {
       L = lik_linreg<LPC, NJ>(up, nj, D, lr, tfm, wave, G, run.log_const);
@@ -1157,7 +1157,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           float* dst = A.B.new_U + o * dps;
 #pragma unroll
           for (int j = 0; j < NJ; ++j)
-            dst[owned_dim<LPC>(h, j)] = U0[j];
+            if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
           if (h == 0) {
             A.B.new_logL[o] = logL0;
             A.B.new_evals[o] = (int32_t)evals;
@@ -1180,7 +1180,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
           float* dst = A.B.new_U + o * dps;
 #pragma unroll
           for (int j = 0; j < NJ; ++j)
-            dst[owned_dim<LPC>(h, j)] = U0[j];
+            if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
           if (h == 0) {
             A.B.new_logL[o] = logL0;
             A.B.new_evals[o] = (int32_t)evals;
@@ -1209,14 +1209,14 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
               float* pd = A.ph_U + ((int64_t)chain * kph + pidx) * dps;
 #pragma unroll
               for (int j = 0; j < NJ; ++j)
-                pd[owned_dim<LPC>(h, j)] = U0[j];
+                if (j < nj) pd[owned_dim<LPC>(h, j)] = U0[j];
               if (h == 0) A.ph_logL[(int64_t)chain * kph + pidx] = L;
             } else {
               const int64_t si = run.samp_off + phantom_base + (int64_t)chain * kph + pidx;
               float* pd = A.B.samp_U + si * dps;
 #pragma unroll
               for (int j = 0; j < NJ; ++j)
-                pd[owned_dim<LPC>(h, j)] = U0[j];
+                if (j < nj) pd[owned_dim<LPC>(h, j)] = U0[j];
               if (h == 0) {
                 A.B.samp_logL[si] = L;
                 A.B.samp_sender[si] = sender;
@@ -1234,7 +1234,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
             float* dst = A.B.new_U + o * dps;
 #pragma unroll
             for (int j = 0; j < NJ; ++j)
-              dst[owned_dim<LPC>(h, j)] = U0[j];
+              if (j < nj) dst[owned_dim<LPC>(h, j)] = U0[j];
             if (h == 0) {
               A.B.new_logL[o] = logL0;
               A.B.new_evals[o] = (int32_t)evals;

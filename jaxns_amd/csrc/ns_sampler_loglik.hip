@@ -35,9 +35,9 @@ __global__ void __launch_bounds__(256) k_loglik(const float* U, int64_t B, doubl
   }
   const uint32_t tfm = transform_mask<LPC, NJ>(FAM == HIPNS_FAM_MVN ? mv.tf : gm.tf, h, nj);
   double L;
-  if (FAM == HIPNS_FAM_MVN)
+  if constexpr (FAM == HIPNS_FAM_MVN)
     L = lik_mvn<LPC, NJ>(up, nj, D, dps, mv, run.log_const, tfm);
-  else if (FAM == HIPNS_FAM_GMM)
+  else if constexpr (FAM == HIPNS_FAM_GMM)
     L = lik_gmm<LPC, NJ>(up, nj, D, gm, tfm);
   else
     L = lik_linreg<LPC, NJ, false, true>(up, nj, D, lr, tfm, wave, wpb, run.log_const);
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) k_loglik_grad(const float* U, int64_t B, 
   }
   const uint32_t tfm = transform_mask<LPC, NJ>(FAM == HIPNS_FAM_MVN ? mv.tf : gm.tf, h, nj);
   double L;
-  if (FAM == HIPNS_FAM_MVN) {
+  if constexpr (FAM == HIPNS_FAM_MVN) {
     float* gscr = lds + gscr_off + (size_t)wave * grad_rows(NJ, LPC) * (CW + 1);
     L = lik_mvn<LPC, NJ, true>(up, nj, D, dps, mv, run.log_const, tfm, gscr, g, true);
   } else {
@@ -106,7 +106,13 @@ template <int FAM>
 static hipError_t launch_loglik_grad_fam(const float* U, int64_t B, double* out, float* grad,
                                          const hipns_run_desc_t* runs, const hipns_buffers_t& Bf, int K, int lpc,
                                          int wpb, int dmax, int gscr_off, size_t lds, hipStream_t s) {
-  if (lpc == 4 || dmax > 64) {
+  if constexpr (FAM == HIPNS_FAM_GMM) {
+    if (lpc == 8 && dmax <= 128) {
+      if (dmax <= 64) return launch_loglik_grad_one<8, 8, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
+      return launch_loglik_grad_one<8, 16, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
+    }
+  }
+  if (lpc == 4 || lpc == 8 || dmax > 64) {
     if (dmax <= 32) return launch_loglik_grad_one<4, 8, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
     if (dmax <= 64) return launch_loglik_grad_one<4, 16, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
     return launch_loglik_grad_one<4, 32, FAM>(U, B, out, grad, runs, Bf, K, wpb, gscr_off, lds, s);
@@ -132,7 +138,13 @@ template <int FAM>
 static hipError_t launch_loglik_fam(const float* U, int64_t B, double* out, const hipns_run_desc_t* runs,
                                    const hipns_buffers_t& Bf, int K, int lpc, int wpb, int dmax, size_t lds,
                                    hipStream_t s, const LinRegLds& lr) {
-  if (lpc == 4 || dmax > 64) {
+  if constexpr (FAM == HIPNS_FAM_GMM) {
+    if (lpc == 8 && dmax <= 128) {
+      if (dmax <= 64) return launch_loglik_one<8, 8, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
+      return launch_loglik_one<8, 16, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
+    }
+  }
+  if (lpc == 4 || lpc == 8 || dmax > 64) {
     if (dmax <= 32) return launch_loglik_one<4, 8, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
     if (dmax <= 64) return launch_loglik_one<4, 16, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);
     return launch_loglik_one<4, 32, FAM>(U, B, out, runs, Bf, K, wpb, lds, s, lr);

@@ -37,7 +37,7 @@ def _fd_grad(spec, U, h=1e-6):
 
 
 @pytest.mark.parametrize("fam,D,lpc", [("mvn", 6, 2), ("mvn", 6, 4), ("mvn", 40, 4), ("mvn", 20, 2),
-                                       ("gmm", 5, 2), ("gmm", 12, 4)])
+                                       ("gmm", 5, 2), ("gmm", 12, 4), ("gmm", 30, 8)])
 def test_loglik_grad_vs_finite_differences(fam, D, lpc):
     """hipns_log_likelihood_grad (the gradient the fused gradient-guided chains use) against central
     differences of the oracle's FP64 likelihood: every component within 2e-5 of the largest component

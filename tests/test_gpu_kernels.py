@@ -250,6 +250,7 @@ def _linreg_posterior_U(lik, tau, n, rng):
 
 @pytest.mark.parametrize("fam,D,lpc,wpb,nd", [("mvn", 10, 2, 2, 0), ("mvn", 32, 2, 2, 0), ("mvn", 32, 4, 2, 0),
                                               ("mvn", 70, 2, 2, 0), ("gmm", 7, 2, 2, 0), ("gmm", 40, 4, 2, 0),
+                                              ("gmm", 7, 8, 2, 0), ("gmm", 40, 8, 1, 0), ("gmm", 100, 8, 2, 0),
                                               ("linreg", 16, 2, 4, 4096), ("linreg", 16, 4, 1, 4096),
                                               ("linreg", 5, 2, 3, 37), ("linreg", 100, 4, 2, 1000)])
 def test_batched_loglik_vs_oracle(fam, D, lpc, wpb, nd):
@@ -298,7 +299,9 @@ def test_batched_loglik_vs_oracle(fam, D, lpc, wpb, nd):
 @pytest.mark.parametrize("fam,D,lpc,k,wpb,faces", [("mvn", 10, 2, 0, 2, False), ("mvn", 32, 2, 2, 2, False),
                                                    ("mvn", 32, 4, 0, 2, False), ("mvn", 5, 4, 1, 2, False),
                                                    ("linreg", 16, 2, 1, 4, False), ("linreg", 40, 4, 0, 3, False),
-                                                   ("mvn", 10, 2, 0, 2, True), ("mvn", 5, 4, 0, 1, True)])
+                                                   ("mvn", 10, 2, 0, 2, True), ("mvn", 5, 4, 0, 1, True),
+                                                   ("gmm", 40, 8, 1, 2, False), ("gmm", 100, 8, 0, 1, False),
+                                                   ("gmm", 6, 4, 0, 2, True), ("mvn", 70, 4, 0, 2, False)])
 def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb, faces):
     """UniDimSliceSampler chains with injected draws: the device trace (t of every pick),
     the accept/reject sequence, eval counts and the accepted points are bit-exact with
@@ -311,6 +314,9 @@ def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb, faces):
     if fam == "mvn":
         model, _, mu, Sig = mvn_problem(D, seed=100 + D)
         spec = mvn_oracle_spec(D, mu, Sig)
+    elif fam == "gmm":
+        model, _, means, sig = gmm_problem(D, seed=100 + D)
+        spec = gmm_oracle_spec(D, means, sig)
     else:
         model, _, lik = linreg_problem(300, D, seed=100 + D, sigma=0.5, tau=1.0)
         spec = linreg_oracle_spec(lik, 1.0)
@@ -330,13 +336,15 @@ def test_slice_chains_injected_bit_exact(fam, D, lpc, k, wpb, faces):
     live_L = b.live_logL[0][:N].cpu().numpy()
     # near-face seeds no longer satisfy the contour they were drawn under; with contour = -inf every
     # slice accepts its first pick, so each slice tests one bracket (the first from the face point)
-    contour = -np.inf if faces else float(live_L[N // 2 - 1])
+    # (the mixture's region above a contour is a few small balls: a lower contour keeps its slices
+    # within the step budget)
+    contour = -np.inf if faces else float(live_L[(N // 4 if fam == "gmm" else N // 2) - 1])
     rng = np.random.default_rng(D + lpc)
     Cn = N // 2
     seeds = np.array([O.seed_index(live_L, contour, u) for u in rng.uniform(size=Cn)], np.int32)
     if faces:
         seeds[:len(face_vals)] = N - 1 - np.arange(len(face_vals))
-    max_steps = 64
+    max_steps = 256 if fam == "gmm" else 64
     z = rng.normal(size=(Cn, ns, D))
     dirs = (z / np.linalg.norm(z, axis=-1, keepdims=True)).astype(np.float32)
     dp = np.zeros((Cn, ns, b.dps), np.float32)
@@ -526,20 +534,24 @@ def test_linreg_fp32_refinement():
     assert abs(z[0, 0] - log_Z) < 4 * z[0, 1] + 0.2
 
 
-def test_uneven_batched_gmm_runs_logZ():
+@pytest.mark.parametrize("lpc", [4, 8])
+def test_uneven_batched_gmm_runs_logZ(lpc):
     """Runs of uneven dimension and live-point count (mixture likelihood, uniform box prior)
-    advanced in one batch: each run's log Z against its own analytic value."""
+    advanced in one batch: each run's log Z against its own analytic value (4 and 8 lanes per
+    chain)."""
     from jaxns_amd.engine import DeviceBatch, RunSpec
-    say("uneven GMM batch")
+    say(f"uneven GMM batch lpc={lpc}")
     rng = np.random.default_rng(123)
     specs, truths = [], []
     for r in range(8):
-        D = int(rng.integers(2, 9))
+        # 8 lanes per chain run where the row stride (64 here) covers the 8-slot instance's 64 dims; at
+        # those dimensions one component (four would lose modes, a bias of the algorithm itself)
+        D = int(rng.integers(2, 9)) if lpc == 4 else int(rng.integers(33, 61))
         N = int(rng.integers(128, 320))
-        model, log_Z, _, _ = gmm_problem(D, seed=1000 + r)
+        model, log_Z, _, _ = gmm_problem(D, seed=1000 + r, K=4 if lpc == 4 else 1)
         specs.append(RunSpec(model=model, num_live_points=N, num_slices=5 * D, max_samples=N * 400, seed=r))
         truths.append(log_Z)
-    b = DeviceBatch(specs, lanes_per_chain=4, waves_per_block=2)
+    b = DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=2)
     b.run(dict(dlogZ=math.log(1.0 + 1e-3), max_samples=1e18), graph_iters=4)
     z = b.log_Z()
     for r in range(8):
