@@ -50,6 +50,11 @@ PEAK_VALU_ISSUE = 256 * 4 * 2.4e9 / 2.0
 # SQ_INSTS_VALU per k_sample launch (rocprofv3 --pmc on the bench's instrumented pass of the same
 # shape), keyed like MEASURED_TRAFFIC
 MEASURED_VALU = {("c2", 1024): 1.1958e10}
+# SQ_INSTS_VALU per likelihood evaluation of the sampling kernel (rocprofv3 --pmc on the instrumented
+# pass: mixture batch of 256 runs at 8 lanes per chain, MVN D=10 batch of 1024 runs at 2 lanes per chain),
+# keyed by (workload, lanes per chain); times a step's evaluations per second it gives the VALU issue
+# rate the step sustains (profiles/k_sample_gmm_mixture_r256.md)
+MEASURED_VALU_PER_EVAL = {("c3", 8): 80.76, ("c1", 2): 21.61}
 # the oracle ensemble of the same workload (tests/golden/oracle_logZ_ensembles.json) beside the
 # device's log Z statistics
 # runs of a step streamed through this many device slots (engine.run_queue) instead of one batch: none
@@ -637,6 +642,12 @@ def secondary(cfg, R, steps, graph_iters, slots=0, groups=1, warmup=1):
         out["roofline_end_to_end"] = dict(bound=wl.bound, achieved=work / scale, peak=wl.peak / scale, unit=wl.unit,
                                           frac=work / wl.peak, kernel=wl.kernel,
                                           note="algorithmic work of the whole timed step over its wall time")
+    vpe = MEASURED_VALU_PER_EVAL.get((cfg, m["lpc"]))
+    if vpe is not None:
+        rate = vpe * m["evals_per_s"]
+        out["valu_issue_end_to_end"] = dict(
+            achieved=rate, peak=PEAK_VALU_ISSUE, unit="wavefront VALU instructions/s", frac=rate / PEAK_VALU_ISSUE,
+            valu_per_eval=vpe, note="measured SQ_INSTS_VALU per evaluation times the step's evaluations per second")
     return out
 
 
@@ -646,7 +657,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--runs", type=int, default=None,
-                    help="runs per step (chosen form); default c2: 1024, c1: 128, c3: 2048, c4: 256, c5: 8")
+                    help="runs per step (chosen form); default c2: 1024, c1: 1024, c3: 2048, c4: 256, c5: 8")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--shells", type=int, default=3, help="c5: shells per partial run")
     ap.add_argument("--full-runs", type=int, default=0, help="c5: also run this many whole runs to termination")
@@ -696,7 +707,7 @@ def main():
         wl.lr_terms = args.lr_terms
     # runs per step: c2 1024 (512: 920 runs/s, 768: 939, 1024: 952 on one box; the shell kernel's tail and
     # the per-run kernels amortise over more runs); c3 1024 (256: 54.7, 512: 65.9, 1024: 74.9)
-    R = args.runs or {"c2": 1024, "c3": 2048, "c4": 256, "c5": 8}.get(args.config, 128)
+    R = args.runs or {"c1": 1024, "c2": 1024, "c3": 2048, "c4": 256, "c5": 8}.get(args.config, 128)
     G = max(1, args.groups)
     RG = (R + G - 1) // G  # runs per group: the batch one launch of the dominant kernel covers
     seed_base = 0
@@ -783,12 +794,14 @@ def main():
     log("cpu baseline", cpu)
     sec = {}
     if not args.no_secondary and args.config == "c2":
-        # (workload, runs per step, timed steps, warm-up steps, concurrent groups); the mixture batch's
-        # 24 s step runs cold (its first-launch costs are below 1 % of the step)
-        for cfg, Rs, st, wu, gr in (("c1", 128, 2, 1, 1), ("c4", 256, 1, 1, 2), ("c3", 2048, 1, 0, 1)):
+        # (line, workload, runs per step, timed steps, warm-up steps, concurrent groups): the mixture
+        # batch at the 256 runs of its configuration and at 2048 (its 12 s step runs cold: first-launch
+        # costs are below 1 % of it)
+        for key, cfg, Rs, st, wu, gr in (("c1", "c1", 1024, 2, 1, 1), ("c4", "c4", 256, 1, 1, 2),
+                                         ("c3", "c3", 256, 2, 1, 1), ("c3_2048", "c3", 2048, 1, 0, 1)):
             t0 = time.perf_counter()
-            sec[cfg] = secondary(cfg, Rs, st, args.graph_iters, slots=QUEUE_SLOTS.get(cfg, 0), groups=gr, warmup=wu)
-            log(f"secondary {cfg}: {sec[cfg]['runs_per_s']:.2f} runs/s in {time.perf_counter() - t0:.1f} s")
+            sec[key] = secondary(cfg, Rs, st, args.graph_iters, slots=QUEUE_SLOTS.get(cfg, 0), groups=gr, warmup=wu)
+            log(f"secondary {key}: {sec[key]['runs_per_s']:.2f} runs/s in {time.perf_counter() - t0:.1f} s")
     out = {
         "metric": "runs/s to |dlogZ|<0.1 nat (chosen batched form, 1 GPU)",
         "value": runs_per_s,
