@@ -130,7 +130,9 @@ typedef struct hipns_buffers {
 } hipns_buffers_t;
 
 /* Kernel shape picked by the autotuner: lanes per chain (2: 32 chains per
- * wavefront with 32x32 MFMA tiles, 4: 16 chains per wavefront with 16x16 tiles),
+ * wavefront with 32x32 MFMA tiles, 4: 16 chains per wavefront with 16x16 tiles,
+ * 8: 8 chains per wavefront, mixture family only, used where the U row stride
+ * covers the 8-lane instance (dps >= 64 for D <= 64, 128 for D <= 128), else 4),
  * wavefronts per workgroup, and the likelihood family. */
 typedef struct hipns_launch {
   int32_t family;
