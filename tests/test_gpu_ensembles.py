@@ -61,7 +61,7 @@ def _device_ensemble(name, R, seed0, groups=1):
             specs.append(RunSpec(model=model, num_live_points=N, num_slices=5 * D, max_samples=N * 300,
                                  seed=seed0 + i))
             truths.append(log_Z)
-        lpc = 2
+        lpc = 8  # the bench's launch shape for the mixture batch (row stride 64 covers the 8-lane instance)
     cut = np.linspace(0, R, groups + 1).round().astype(int)
     batches = [DeviceBatch(specs[cut[g]:cut[g + 1]], lanes_per_chain=lpc, waves_per_block=2) for g in range(groups)]
     terms = [dict(dlogZ=math.log(1.0 + 1e-3), max_samples=float(max(s.max_samples for s in b.specs)))

@@ -81,18 +81,25 @@ def test_loglik_grad_vs_finite_differences(fam, D, lpc):
     assert np.all(gd.cpu().numpy()[:, D:] == 0)
 
 
-@pytest.mark.parametrize("which", ["mvn", "gmm"])
-def test_gradient_guided_fused_logZ(which):
+@pytest.mark.parametrize("which,lpc", [("mvn", 2), ("gmm", 2), ("gmm8", 8)])
+def test_gradient_guided_fused_logZ(which, lpc):
     """NestedSampler(gradient_guided=True) on a fused family runs the fused kernel (sampler code 2): log Z
     against the analytic value, no step-cap hits, and every slice-sampled point spent at least two
     evaluations per slice (its accepted proposal plus the gradient at it)."""
-    say(f"gradient-guided fused {which}")
+    say(f"gradient-guided fused {which} lpc={lpc}")
     if which == "mvn":
         model, log_Z, _, _ = mvn_problem(8, seed=4)
-        ns = J.NestedSampler(model, num_live_points=200, s=3, max_samples=200 * 150, gradient_guided=True)
-    else:
+        ns = J.NestedSampler(model, num_live_points=200, s=3, max_samples=200 * 150, gradient_guided=True,
+                             lanes_per_chain=lpc)
+    elif which == "gmm":
         model, log_Z, _, _ = gmm_problem(3, seed=7)
-        ns = J.NestedSampler(model, num_live_points=300, s=4, max_samples=300 * 200, gradient_guided=True)
+        ns = J.NestedSampler(model, num_live_points=300, s=4, max_samples=300 * 200, gradient_guided=True,
+                             lanes_per_chain=lpc)
+    else:
+        # 8 lanes per chain need a row stride of 64: D = 40, one component
+        model, log_Z, _, _ = gmm_problem(40, seed=7, K=1)
+        ns = J.NestedSampler(model, num_live_points=300, s=2, max_samples=300 * 300, gradient_guided=True,
+                             lanes_per_chain=lpc)
     reason, state = ns(5)
     b = state.batch
     assert not b.external and int(b.runs_host[0].sampler) == 2
