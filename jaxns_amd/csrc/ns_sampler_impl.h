@@ -227,8 +227,23 @@ __device__ __forceinline__ void lr_bf16_tiles(const LinRegLds& P, const bf16x8_t
       xh[gg] = __builtin_bit_cast(bf16x8_t, *(const uint4*)lr_panel_ptr<LPC>(P.xh, P.npad, gg, row, h));
       xl[gg] = __builtin_bit_cast(bf16x8_t, *(const uint4*)lr_panel_ptr<LPC>(P.xl, P.npad, gg, row, h));
     }
+    // the lane's rows of the tile come in aligned runs of four (tile_row): r0 in 16-byte loads
+    if constexpr (LPC == 2) {
 #pragma unroll
-    for (int i = 0; i < NR; ++i) rv[i] = P.r0[tile_row(tl, i)];
+      for (int q = 0; q < 4; ++q) {
+        const float4 v = *(const float4*)(P.r0 + tl * 32 + 8 * q + 4 * h);
+        rv[4 * q] = v.x;
+        rv[4 * q + 1] = v.y;
+        rv[4 * q + 2] = v.z;
+        rv[4 * q + 3] = v.w;
+      }
+    } else {
+      const float4 v = *(const float4*)(P.r0 + tl * 16 + 4 * h);
+      rv[0] = v.x;
+      rv[1] = v.y;
+      rv[2] = v.z;
+      rv[3] = v.w;
+    }
   };
   // groups per pass over a tile: LPC=2 accumulators are 16 VGPRs each, so two at a time
   constexpr int QB = (LPC == 2 && GG > 2) ? 2 : GG;
