@@ -57,8 +57,10 @@ typedef struct hipns_run_desc {
   int32_t k_phantom;        /* phantom samples kept per chain (k) */
   int32_t midpoint_shrink;  /* 1: midpoint (alpha) shrink, 0: shrink to rejected point */
   int32_t sampler;          /* 0: UniDimSliceSampler, 1: UniformSampler (rejection), 2: UniDimSliceSampler
-                             * with gradient-guided directions (uni_slice_sampler.py:255-271), d logL / dU
-                             * computed in the sampling kernel (MVN and mixture families) */
+                             * with gradient-guided directions (uni_slice_sampler.py:255-271), 3: with
+                             * gradient_slice (each slice along the unit gradient at its seed over [0, right],
+                             * uni_slice_sampler.py:202-214), 4: both; for 2-4 d logL / dU is computed in the
+                             * sampling kernel (MVN and mixture families only) */
   int32_t max_uniform_evals;/* UniformSampler cap (100 in the reference) */
   int64_t max_samples;      /* sample-collection length */
   int64_t live_off;         /* offset into the live-point arrays (points) */
@@ -359,8 +361,8 @@ int hipns_log_likelihood(const float* U_dev, int64_t B, double* logL_dev, const 
                          void* stream);
 
 /* The batched likelihood with its gradient d logL / dU (the gradient of Model.forward that the
- * gradient-guided slice sampler asks for, uni_slice_sampler.py:255-258), computed as the sampling
- * kernel computes it for sampler code 2: MVN, -(W^T r) (.) dq/du with r = W q(U) + b; mixture, the
+ * gradient-guided and gradient-slice samplers ask for, uni_slice_sampler.py:204,258), computed as the
+ * sampling kernel computes it for sampler codes 2-4: MVN, -(W^T r) (.) dq/du with r = W q(U) + b; mixture, the
  * softmax-weighted component gradients times the prior scale and dq/du.  grad_dev has row stride
  * bufs->dps (entries d < D written); FP32 products, as in the kernel.  MVN and mixture families. */
 int hipns_log_likelihood_grad(const float* U_dev, int64_t B, double* logL_dev, float* grad_dev,

@@ -385,11 +385,11 @@ int hipns_shell_sample(const hipns_run_desc_t* runs_dev, hipns_run_state_t* stat
   A.gmm_K = launch->gmm_K;
   int max_slices = 0;
   for (int r = 0; r < R; ++r) max_slices = runs_host[r].num_slices > max_slices ? runs_host[r].num_slices : max_slices;
-  if (sampler == 2 && launch->family == HIPNS_FAM_LINREG)
-    return fail(HIPNS_EINVAL, "gradient-guided shells (sampler 2) run the MVN and mixture families; the regression "
-                              "family steps through hipns_ext_step");
-  if (sampler < 0 || sampler > 2) return fail(HIPNS_EINVAL, "sampler must be 0, 1 or 2");
-  const int mode = sampler == 1 ? hipns::MODE_UNIF : (sampler == 2 ? hipns::MODE_GSLICE : hipns::MODE_SLICE);
+  if (sampler < 0 || sampler > 4) return fail(HIPNS_EINVAL, "sampler must be 0, 1, 2, 3 or 4");
+  if (sampler >= 2 && launch->family == HIPNS_FAM_LINREG)
+    return fail(HIPNS_EINVAL, "gradient slice shells (sampler %d) run the MVN and mixture families; gradient-guided "
+                              "regression shells step through hipns_ext_step", sampler);
+  const int mode = sampler == 1 ? hipns::MODE_UNIF : (sampler >= 2 ? hipns::MODE_GSLICE : hipns::MODE_SLICE);
   return sampler_launch(mode, launch, A, R, maxS, maxD, bufs->dps, max_slices, as_stream(stream));
 }
 

@@ -906,6 +906,20 @@ __device__ __forceinline__ void reflect_direction(float (&nv)[NJ], float (&g)[NJ
   for (int j = 0; j < NJ; ++j) nv[j] = nv[j] * inv;
 }
 
+// gradient slice direction (uni_slice_sampler.py:202-214): the unit gradient g / |g|, unless |g| is 0 or
+// not finite (then false, and the slice keeps its own direction and full bracket)
+template <int LPC, int NJ>
+__device__ __forceinline__ bool gradient_direction(float (&nv)[NJ], const float (&g)[NJ]) {
+  float gg2 = 0.0f;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) gg2 = fmaf(g[j], g[j], gg2);
+  const float gn = __builtin_sqrtf(chain_sum<LPC>(gg2));
+  if (!(gn > 0.0f) || !__builtin_isfinite(gn)) return false;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) nv[j] = g[j] / gn;
+  return true;
+}
+
 // a uniform point of the unit cube for attempt `attempt` of a chain (Model.sample_U; padding 0.5)
 template <int LPC, int NJ>
 __device__ __forceinline__ void uniform_point(float (&up)[NJ], int chain, uint32_t it32, int attempt, int D, int nj,
@@ -926,9 +940,12 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   // the regression family multiplies the data rows against every chain group of the workgroup
   // cooperatively, so its wavefronts step together (workgroup-uniform loop)
   constexpr bool COOP = (FAM == HIPNS_FAM_LINREG);
-  // gradient-guided slice chains (uni_slice_sampler.py:255-271): after each accepted slice the chain
-  // spends one step (one likelihood evaluation) on d logL / dU at the accepted point, and the next
-  // direction is the reflection of the last one about it
+  // gradient slice chains, run code sampler 2, 3 or 4.  gradient_guided (2 and 4,
+  // uni_slice_sampler.py:255-271): after each accepted slice the chain spends one step (one likelihood
+  // evaluation) on d logL / dU at the accepted point, and the next direction is the reflection of the
+  // last one about it.  gradient_slice (3 and 4, uni_slice_sampler.py:202-214): every slice runs along
+  // the unit gradient at its seed point over the half bracket [0, right]; the gradient step before slice
+  // s > 0 is the step after slice s - 1, at the same point.
   constexpr bool GG = (MODE == MODE_GSLICE);
   constexpr bool SL = (MODE == MODE_SLICE || MODE == MODE_INJ || GG);  // slice chains
   constexpr bool DRAWN = (MODE == MODE_SLICE || GG);                   // slice chains on Philox draws
@@ -1028,9 +1045,11 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
   float4* ring_base = nullptr;
   if (RB > 0) ring_base = (float4*)(lds + A.ring_off) + (size_t)wave * ring * RSLOT + l;
   int gen = 0, gslot = 0;  // next slice to draw and its slot
-  // gradient-guided: the wavefront's LDS scratch for d logL / dU, and the state of the gradient step
+  // gradient slice chains: the wavefront's LDS scratch for d logL / dU, and the state of the gradient step
   float* gscr = GG ? lds + A.ring_off + (size_t)wave * grad_rows(NJ, LPC) * (CW + 1) : nullptr;
-  bool need_grad = false, dir_ready = false;
+  const bool gs_on = GG && run.sampler >= 3;  // gradient_slice
+  const bool gg_on = GG && run.sampler != 3;  // gradient_guided
+  bool need_grad = false, dir_ready = false, half = false;
   float gv[NJ];
 
   // ---- begin a slice: direction and perfect bracket (the pick follows in the step)
@@ -1038,7 +1057,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
     k = 0;
     alpha = alpha_tab[si];
     if (GG && dir_ready) {
-      dir_ready = false;  // the reflected direction of the gradient step
+      dir_ready = false;  // the direction of the gradient step (reflected, or the unit gradient)
     } else if (RB > 0 && ring > 0) {
       // slice si = s is in the ring (gen > s), gen - s slots behind the next free one
       int rslot = gslot - (gen - si);
@@ -1063,12 +1082,17 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
       slice_direction<LPC, NJ>(nv, chain, it32, si, D, nj, h, k0, k1);
     }
     slice_bracket<LPC, NJ>(U0, nv, left, right);
+    if (GG && half) {
+      left = 0.0f;  // climb the gradient: the bracket's negative side is dropped (uni_slice_sampler.py:212)
+      half = false;
+    }
   };
 
   auto draw_uniform_point = [&](int attempt) { uniform_point<LPC, NJ>(up, chain, it32, attempt, D, nj, h, k0, k1); };
 
   bool need_begin = SL && alive;
   bool need_shrink = false;
+  need_grad = gs_on && alive;  // gradient_slice: the first slice needs the gradient at the seed point
 
   // the shell's phantoms are appended after its discarded points (add_phantom_samples_to_state,
   // sharded_static.py:181-207), chain-major, k per chain
@@ -1162,10 +1186,14 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
         }
       }
       if (GG && need_grad) {
-        // the gradient step (counted as one evaluation, as grad_fn is): finish after the last slice,
-        // else reflect the direction about the gradient, or draw a new one where the gradient is
-        // below 1e-10 or not finite (uni_slice_sampler.py:255-271)
+        // the gradient step, counted as one evaluation per grad_fn call of the reference: with both
+        // gradient_guided and gradient_slice, the step between two slices stands for both calls at the
+        // accepted point.  Finish after the last slice; else, gradient_slice, the next slice runs along
+        // the unit gradient (or its own direction where the gradient is 0 or not finite:
+        // uni_slice_sampler.py:202-214); gradient_guided, reflect the direction about the gradient, or
+        // draw a new one where the gradient is below 1e-10 or not finite (uni_slice_sampler.py:255-271)
         need_grad = false;
+        if (gs_on && gg_on && s > 0 && s < num_slices) ++evals;
         if (s == num_slices) {
           alive = false;
           const int64_t o = run.chain_off + chain;
@@ -1177,6 +1205,12 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
             A.B.new_logL[o] = logL0;
             A.B.new_evals[o] = (int32_t)evals;
           }
+        } else if (gs_on) {
+          // a gradient below 1e-10 but not 0 is reflected about by gradient_guided, and that direction
+          // then gives way to the unit gradient; a zero or non-finite one leaves the slice's own draw
+          dir_ready = gradient_direction<LPC, NJ>(nv, gv);
+          half = dir_ready;
+          need_begin = true;
         } else {
           reflect_direction<LPC, NJ>(nv, gv, chain, it32, s, D, nj, h, k0, k1);
           dir_ready = true;
@@ -1241,8 +1275,10 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A) {
             }
           }
           ++s;
-          if (GG) {
-            need_grad = true;  // every accepted slice asks for the gradient, the last one included
+          if (GG && (gg_on || s < num_slices)) {
+            // gradient_guided: every accepted slice asks for the gradient, the last one included;
+            // gradient_slice: every slice but the first starts with the gradient at its seed
+            need_grad = true;
           } else if (s == num_slices) {
             alive = false;
             const int64_t o = run.chain_off + chain;

@@ -94,13 +94,23 @@ def seed_words(key) -> tuple:
     return v & 0xFFFFFFFF, (v >> 32) & 0xFFFFFFFF
 
 
+def slice_sampler_code(gradient_guided: bool, gradient_slice: bool) -> int:
+    """The run descriptor's sampler code of a UniDimSliceSampler: 0 plain, 2 gradient_guided, 3
+    gradient_slice, 4 both."""
+    if gradient_slice:
+        return 4 if gradient_guided else 3
+    return 2 if gradient_guided else 0
+
+
 class DeviceBatch:
     """R independent runs sharing one likelihood family, advanced together."""
 
     def __init__(self, specs: List[RunSpec], *, lanes_per_chain: int = 2, waves_per_block: int = 4,
                  max_steps: int = 512, device: str = "cuda", lr_terms: int = 3, gradient_guided: bool = False,
-                 refine_fp32: bool = False, slot_capacity: Optional[dict] = None):
-        """slot_capacity (keys N, max_samples, D, num_slices, k_phantom): every run's buffers are sized for
+                 refine_fp32: bool = False, slot_capacity: Optional[dict] = None, gradient_slice: bool = False):
+        """gradient_slice: every slice along the unit gradient at its seed over the half bracket [0, right]
+        (UniDimSliceSampler(gradient_slice=True), uni_slice_sampler.py:202-214), MVN and mixture families.
+        slot_capacity (keys N, max_samples, D, num_slices, k_phantom): every run's buffers are sized for
         that capacity, so a finished run's slot can take another run of at most that size (run_queue);
         the shell launches are shaped for the capacity and read each run's own sizes on the device."""
         torch = _torch()
@@ -111,11 +121,14 @@ class DeviceBatch:
         fam = specs[0].model.log_likelihood
         self.family = getattr(fam, "family", None)
         self.gradient_guided = bool(gradient_guided)
+        self.gradient_slice = bool(gradient_slice)
+        if self.gradient_slice and self.family not in (_lib.FAM_MVN, _lib.FAM_GMM):
+            raise NotImplementedError("gradient_slice runs in the fused kernel, for the MVN and mixture families")
         # gradient-guided slicing needs d logL / dU at accepted points: the MVN and mixture kernels compute
-        # it themselves (sampler code 2); the regression family takes it from torch autograd through the
-        # stepwise path
+        # it themselves (sampler code 2, or 4 with gradient_slice); the regression family takes it from
+        # torch autograd through the stepwise path
         self.external = self.family is None or (self.gradient_guided and self.family == _lib.FAM_LINREG)
-        self.slice_sampler = 2 if (self.gradient_guided and not self.external) else 0
+        self.slice_sampler = slice_sampler_code(self.gradient_guided and not self.external, self.gradient_slice)
         if self.external:
             # a torch callable: evaluated between one-step sampler launches (hipns_ext_step)
             fn = fam.torch_log_prob if self.family is not None else fam

@@ -18,6 +18,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import RunState, check, lib, ptr
+from .engine import slice_sampler_code
 from .types import TerminationRegister
 
 
@@ -161,19 +162,20 @@ class _DeviceSamplerBase(AbstractSampler):
 class UniDimSliceSampler(_DeviceSamplerBase):
     """samplers/uni_slice_sampler.py:276-441: num_slices slices per sample along random directions (or,
     gradient_guided, each direction after the first reflected about d logL / dU at the last accepted
-    point), the bracket the chord of the unit cube (perfect), midpoint (alpha) shrinkage, the last
-    num_phantom_save intermediate points kept as phantom samples."""
+    point, or, gradient_slice, each slice along the unit gradient at its seed over [0, right]), the
+    bracket the chord of the unit cube (perfect), midpoint (alpha) shrinkage, the last num_phantom_save
+    intermediate points kept as phantom samples."""
     _sampler_code = 0
 
     def __init__(self, model, num_slices: int, num_phantom_save: int, midpoint_shrink: bool, perfect: bool = True,
                  gradient_slice: bool = False, gradient_guided: bool = False, **launch_kw):
         if not perfect:
             raise NotImplementedError("Step out not implemented.")  # as the reference (uni_slice_sampler.py:224)
-        if gradient_slice:
-            raise NotImplementedError("gradient_slice is not built (NestedSampler never sets it)")
-        if gradient_guided and getattr(model.log_likelihood, "family", None) not in (_lib.FAM_MVN, _lib.FAM_GMM):
-            raise NotImplementedError("gradient-guided samples in one launch need the MVN or mixture family; "
-                                      "other likelihoods run through NestedSampler(gradient_guided=True)")
+        if (gradient_guided or gradient_slice) and \
+                getattr(model.log_likelihood, "family", None) not in (_lib.FAM_MVN, _lib.FAM_GMM):
+            raise NotImplementedError("gradient-guided or gradient-slice samples in one launch need the MVN or "
+                                      "mixture family; other likelihoods run gradient-guided through "
+                                      "NestedSampler(gradient_guided=True)")
         if num_slices < 1:
             raise ValueError(f"num_slices should be >= 1, got {num_slices}.")
         if not (0 <= num_phantom_save < num_slices):
@@ -181,9 +183,12 @@ class UniDimSliceSampler(_DeviceSamplerBase):
         super().__init__(model, num_phantom_save=num_phantom_save, midpoint_shrink=midpoint_shrink,
                          num_slices=num_slices, **launch_kw)
         self.gradient_guided = bool(gradient_guided)
-        # sampler code 2: after each accepted slice, d logL / dU at the accepted point (one more
-        # evaluation) and the next direction reflected about it (uni_slice_sampler.py:255-271)
-        self._sampler_code = 2 if self.gradient_guided else 0
+        self.gradient_slice = bool(gradient_slice)
+        # gradient_guided (code 2): after each accepted slice, d logL / dU at the accepted point (one more
+        # evaluation) and the next direction reflected about it (uni_slice_sampler.py:255-271);
+        # gradient_slice (code 3, 4 with both): each slice along the unit gradient at its seed (one more
+        # evaluation) over [0, right] (uni_slice_sampler.py:202-214)
+        self._sampler_code = slice_sampler_code(self.gradient_guided, self.gradient_slice)
 
     def num_phantom(self) -> int:
         return self.num_phantom_save

@@ -37,13 +37,14 @@ def _fd_grad(spec, U, h=1e-6):
 
 
 @pytest.mark.parametrize("fam,D,lpc", [("mvn", 6, 2), ("mvn", 6, 4), ("mvn", 40, 4), ("mvn", 20, 2),
-                                       ("gmm", 5, 2), ("gmm", 12, 4), ("gmm", 30, 8)])
+                                       ("gmm", 5, 2), ("gmm", 12, 4), ("gmm", 40, 8)])
 def test_loglik_grad_vs_finite_differences(fam, D, lpc):
     """hipns_log_likelihood_grad (the gradient the fused gradient-guided chains use) against central
     differences of the oracle's FP64 likelihood: every component within 2e-5 of the largest component
     of its row (FP32 products and the FP32 ndtri; about 1e-6 measured), and log L within 1e-5
     relative.  The MVN cases cover
-    one and several 16/32-row output blocks of the second GEMM (W^T r)."""
+    one and several 16/32-row output blocks of the second GEMM (W^T r); the mixture at D = 40 (row stride
+    64) runs at 8 lanes per chain."""
     import torch
     from jaxns_amd.engine import DeviceBatch, RunSpec
     say(f"loglik grad {fam} D={D} lpc={lpc}")
@@ -154,3 +155,110 @@ def test_gradient_guided_fused_and_stepwise_agree():
         assert abs(z.mean() - log_Z) < 4 * se + 0.05
     assert abs(zf.mean() - zs.mean()) < 4 * math.sqrt(sf ** 2 + ss ** 2) + 0.02
     assert abs(ef.mean() / es.mean() - 1) < 0.05
+
+
+def _one_seed_live_points(spec, D, rng, n=50):
+    """n uniform live points and a contour between the best two: every chain starts from the best one."""
+    U = rng.uniform(0.02, 0.98, size=(n, D)).astype(np.float32).astype(np.float64)
+    L = spec.log_likelihood(U)
+    order = np.argsort(L)
+    return U, L, U[order[-1]], float(L[order[-2]])
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+@pytest.mark.parametrize("fam,D,lpc", [("mvn", 6, 2), ("mvn", 20, 4), ("gmm", 5, 2), ("gmm", 40, 8)])
+def test_gradient_slice_moves_up_the_gradient(fam, D, lpc):
+    """UniDimSliceSampler(gradient_slice=True) (uni_slice_sampler.py:202-214): every slice runs along the
+    unit gradient at its seed over [0, right].  From a single seed point, with two slices and one phantom
+    per sample, the phantom (the point after the first slice) lies on the ray from the seed along the
+    central-difference gradient of the oracle's FP64 likelihood, and the sample on the ray from the phantom
+    along the gradient there: 1 - cos < 1e-6 and a positive step.  Each sample spends one evaluation per
+    slice on the gradient (at least 4 in all); with gradient_guided as well, the samples are the same bit
+    for bit and every sample counts one more evaluation per slice (the reference's second grad_fn call at
+    each accepted point)."""
+    from jaxns_amd.samplers import EphemeralState, LivePointCollection, UniDimSliceSampler
+    say(f"gradient slice {fam} D={D} lpc={lpc}")
+    rng = np.random.default_rng(100 + D)
+    if fam == "mvn":
+        model, _, mu, Sig = mvn_problem(D, seed=D)
+        spec = mvn_oracle_spec(D, mu, Sig)
+    else:
+        model, _, means, sig = gmm_problem(D, seed=D)  # D = 40: row stride 64, 8 lanes per chain
+        spec = gmm_oracle_spec(D, means, sig)
+    U, L, U0, contour = _one_seed_live_points(spec, D, rng)
+    lp = LivePointCollection(sender_node_idx=np.zeros(len(L), np.int64), U_sample=U, log_L_constraint=np.zeros(len(L)),
+                             log_L=L, num_likelihood_evaluations=np.ones(len(L), np.int64))
+    n = 256
+    out = {}
+    for gg in (False, True):
+        sampler = UniDimSliceSampler(model, num_slices=2, num_phantom_save=1, midpoint_shrink=True, gradient_slice=True,
+                                     gradient_guided=gg, lanes_per_chain=lpc)
+        assert sampler._sampler_code == (4 if gg else 3)
+        state = sampler.pre_process(EphemeralState(key=0, live_points_collection=lp))
+        out[gg] = sampler.get_samples((41, 3), contour, state, n)
+    (s, ph), (s2, ph2) = out[False], out[True]
+    assert np.all(s.log_L > contour) and np.all(ph.log_L > contour)
+    assert np.all((s.U_sample >= 0) & (s.U_sample <= 1)) and np.all((ph.U_sample >= 0) & (ph.U_sample <= 1))
+    g0 = _unit(_fd_grad(spec, U0[None, :])[0])
+    gp = _unit(_fd_grad(spec, ph.U_sample))
+    d1 = ph.U_sample - U0
+    d2 = s.U_sample - ph.U_sample
+    n1, n2 = np.linalg.norm(d1, axis=1), np.linalg.norm(d2, axis=1)
+    m1, m2 = n1 > 1e-3, n2 > 1e-3  # steps long enough that FP32 rounding of the points leaves the angle
+    say(f"  steps > 1e-3: {m1.sum()} / {n}, {m2.sum()} / {n}; evals {s.num_likelihood_evaluations.min()}.."
+        f"{s.num_likelihood_evaluations.max()}")
+    assert m1.sum() > n // 2 and m2.sum() > n // 2
+    cos1 = (d1[m1] @ g0) / n1[m1]
+    cos2 = np.sum(d2[m2] * gp[m2], axis=1) / n2[m2]
+    say(f"  1 - cos: first slice {np.max(1 - cos1):.2e}, second {np.max(1 - cos2):.2e}")
+    assert np.all(1 - cos1 < 1e-6) and np.all(1 - cos2 < 1e-6)
+    assert np.all(s.num_likelihood_evaluations >= 4)
+    # gradient_guided on top: the same chains, one more evaluation per slice
+    assert np.array_equal(s2.U_sample, s.U_sample) and np.array_equal(s2.log_L, s.log_L)
+    assert np.array_equal(ph2.U_sample, ph.U_sample)
+    assert np.all(s2.num_likelihood_evaluations == s.num_likelihood_evaluations + 2)
+
+
+def test_gradient_slice_whole_runs():
+    """Whole runs with gradient_slice (DeviceBatch(gradient_slice=True), sampler code 3) and with
+    gradient_slice and gradient_guided (code 4) on four seeds of an MVN problem: both terminate with no
+    step-cap hits and collect the same samples bit for bit; per sample, the evaluation counts are equal
+    for the initial points and the phantoms and one per slice apart for every slice sample; the best sample
+    ends within 0.5 nat of the likelihood's peak (the sampler is meant to climb)."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    say("gradient slice whole runs")
+    D, N = 6, 200
+    model, _, mu, Sig = mvn_problem(D, seed=5)
+    ns = 2 * D
+    specs = [RunSpec(model=model, num_live_points=N, num_slices=ns, k_phantom=1, max_samples=N * 150, seed=900 + i)
+             for i in range(4)]
+    res = {}
+    for gg in (False, True):
+        b = DeviceBatch(specs, lanes_per_chain=2, waves_per_block=2, gradient_slice=True, gradient_guided=gg)
+        assert not b.external and int(b.runs_host[0].sampler) == (4 if gg else 3)
+        b.run(dict(dlogZ=math.log(1.0 + 1e-2), max_samples=1e18), init_efficiency_threshold=0.0)
+        assert int(b.cap_hits().sum()) == 0
+        res[gg] = (b, [b.sample_collection(r) for r in range(len(specs))], b.states())
+    (b, cols, sts), (_, cols2, sts2) = res[False], res[True]
+    peak = float(model.log_likelihood.log_const)  # log L at x = mu
+    for r in range(len(specs)):
+        c, c2 = cols[r], cols2[r]
+        m = int(sts[r].num_samples)
+        assert int(sts2[r].num_samples) == m
+        assert np.array_equal(c.U_samples[:m], c2.U_samples[:m]) and np.array_equal(c.log_L[:m], c2.log_L[:m])
+        diff = c2.num_likelihood_evaluations[:m].astype(np.int64) - c.num_likelihood_evaluations[:m]
+        assert set(np.unique(diff)) <= {0, ns}
+        # the collection holds the dead points of every shell, their phantoms and finally the live set:
+        # each initial point once (one evaluation: the MVN log L is finite everywhere), every other
+        # non-phantom entry a slice sample (ns gradient steps and at least ns proposals)
+        ph = c.phantom[:m]
+        ev = c.num_likelihood_evaluations[:m]
+        init = ~ph & (ev == 1)
+        slice_samples = ~ph & ~init
+        assert init.sum() == N and np.all(ev[slice_samples] >= 2 * ns)
+        assert np.all(diff[ph] == 0) and np.all(diff[init] == 0) and np.all(diff[slice_samples] == ns)
+        say(f"  run {r}: {m} samples, best log L {c.log_L[:m].max():.4f}, peak {peak:.4f}")
+        assert c.log_L[:m].max() > peak - 0.5

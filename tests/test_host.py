@@ -70,6 +70,27 @@ def test_public_defaults():
     assert t["dlogZ"] == pytest.approx(np.log(1 + 1e-3)) and t["max_samples"] == 30000
 
 
+def test_slice_sampler_codes_and_gradient_slice_scope():
+    # UniDimSliceSampler flags (uni_slice_sampler.py:276-320) -> the run descriptor's sampler code
+    from jaxns_amd.engine import DeviceBatch, slice_sampler_code
+    from jaxns_amd.samplers import UniDimSliceSampler
+    assert [slice_sampler_code(gg, gs) for gg, gs in ((0, 0), (1, 0), (0, 1), (1, 1))] == [0, 2, 3, 4]
+    model, _, _, _ = mvn_problem(4)
+    assert UniDimSliceSampler(model, 8, 1, True, gradient_slice=True)._sampler_code == 3
+    assert UniDimSliceSampler(model, 8, 1, True, gradient_slice=True, gradient_guided=True)._sampler_code == 4
+    with pytest.raises(NotImplementedError):
+        UniDimSliceSampler(model, 8, 1, True, perfect=False)
+    # gradient_slice runs in the fused kernel only (the MVN and mixture families)
+    lin, _ = linreg_problem(64, 3)[:2]
+    with pytest.raises(NotImplementedError):
+        UniDimSliceSampler(lin, 8, 0, True, gradient_slice=True)
+    with pytest.raises(NotImplementedError):
+        DeviceBatch([RunSpec(model=lin, num_live_points=16, num_slices=4, max_samples=1000)], gradient_slice=True)
+    cb = J.Model(model.prior_model, lambda x: -0.5 * (x * x).sum(-1))
+    with pytest.raises(NotImplementedError):
+        DeviceBatch([RunSpec(model=cb, num_live_points=16, num_slices=4, max_samples=1000)], gradient_slice=True)
+
+
 def test_term_cond_combinators():
     a = J.TerminationCondition(dlogZ=0.1)
     b = J.TerminationCondition(max_samples=10)
