@@ -411,7 +411,8 @@ int hipns_ndtri_f32(const float* u_dev, int64_t n, float* z_dev, void* stream);
 
 /* The slice bracket's quotient a / n (reciprocal + residual corrections, no operand scaling),
  * elementwise, for tests that it equals IEEE binary32 division over the bracket's operand range
- * (uni_slice_sampler.py:41-64 divides (0 - x) and (1 - x) by the direction). */
+ * (uni_slice_sampler.py:41-64 divides (0 - x) and (1 - x) by the direction).  a_dev = NULL: the
+ * reciprocal y of n that the quotient starts from, for tests that it is 1 / n correctly rounded. */
 int hipns_slice_quotients(const float* a_dev, const float* n_dev, int64_t count, float* q_dev, void* stream);
 
 /* The seed point chain i of a shell picks in iteration `iteration` of a run keyed (key0, key1), i in

@@ -215,7 +215,10 @@ __global__ void k_ndtri(const float* u, int64_t n, float* z) {
 
 __global__ void k_slice_quotients(const float* a, const float* n, int64_t count, float* q) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) q[i] = hipns::quot_rn(a[i], n[i], hipns::recip_rn(n[i]));
+  if (i < count) {
+    const float y = hipns::recip_rn(n[i]);
+    q[i] = a ? hipns::quot_rn(a[i], n[i], y) : y;
+  }
 }
 
 // X[b, i] = loc[i] + sum_{j<=i} tril[i, j] q_j(U[b, j]) when tril != NULL, else loc[i] + scale[i] q_i(U[b, i])
@@ -690,7 +693,7 @@ int hipns_ndtri_f32(const float* u_dev, int64_t n, float* z_dev, void* stream) {
 int hipns_slice_quotients(const float* a_dev, const float* n_dev, int64_t count, float* q_dev, void* stream) {
   if (count < 0) return fail(HIPNS_EINVAL, "count < 0");
   if (count == 0) return HIPNS_OK;
-  if (!a_dev || !n_dev || !q_dev) return fail(HIPNS_EINVAL, "null pointer");
+  if (!n_dev || !q_dev) return fail(HIPNS_EINVAL, "null pointer");
   hipLaunchKernelGGL(k_slice_quotients, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, as_stream(stream), a_dev,
                      n_dev, count, q_dev);
   HIPNS_CHECK_LAUNCH("k_slice_quotients");

@@ -860,9 +860,10 @@ __device__ __forceinline__ void slice_bracket(const float (&U0)[NJ], const float
     amin = min(amin, min(__float_as_int(u0), __float_as_int(a1)));
     const float y = recip_rn(n);
     const float t1 = quot_rn(a1, n, y), t0 = quot_rn(a0, n, y);
-    const bool pos = n > 0.0f;
-    rmin = fminf(rmin, pos ? t1 : t0);
-    lmax = fmaxf(lmax, pos ? t0 : t1);
+    // a1 >= 0 >= a0, so the quotient of the sign of n is the right candidate and the other the left one
+    // (n = +-0: both NaN, which fminf / fmaxf skip)
+    rmin = fminf(rmin, fmaxf(t0, t1));
+    lmax = fmaxf(lmax, fminf(t0, t1));
   }
   if (__any(amin < 0x21800000)) {  // 0x21800000 = 2^-60
     rmin = __builtin_inff();

@@ -61,7 +61,7 @@ def test_ndtri_f32_accuracy():
 
 
 def test_slice_quotient_matches_ieee_division():
-    """The slice bracket's quotient (hardware reciprocal, one Newton step, two residual corrections)
+    """The slice bracket's quotient (hardware reciprocal, one Newton step, one residual correction)
     is bit-identical to IEEE binary32 division: every denominator of six whole binades (all 2^23
     mantissas, random signs) against numerators 1 - u0 and -u0, then 2^24 log-uniform pairs over the
     bracket's range |a| in [2^-60, 2], |n| in [2^-94, 1]."""
@@ -93,6 +93,35 @@ def test_slice_quotient_matches_ieee_division():
     a = (np.exp2(rng.uniform(-60, 1, size=M)) * rng.choice([-1.0, 1.0], size=M)).astype(np.float32)
     n = (np.exp2(rng.uniform(-94, 0, size=M)) * rng.choice([-1.0, 1.0], size=M)).astype(np.float32)
     assert run(a, n) == 0
+
+
+def test_slice_reciprocal_is_correctly_rounded():
+    """The reciprocal the slice bracket's quotient starts from (hardware reciprocal and one Newton step)
+    equals 1 / n rounded to nearest for every binary32 n of magnitude in [2^-94, 1] (all 2^23 mantissas
+    of 94 binades and n = 1, positive; four binades and n = -1 negative): with y = RN(1/n), one
+    residual correction of q = a y gives RN(a / n) (P. Markstein, IBM J. Res. Dev. 34(1), 1990)."""
+    torch = _torch()
+    mant = torch.arange(1 << 23, dtype=torch.int64, device="cuda")
+    y = torch.empty(1 << 23, dtype=torch.float32, device="cuda")
+    bad = 0
+    cases = [(e, 1.0) for e in range(-94, 0)] + [(e, -1.0) for e in (-1, -2, -23, -94)]
+    for e, sign in cases:
+        bits = ((127 + e) << 23) + mant
+        n = (bits.to(torch.int32).view(torch.float32) * sign).contiguous()
+        check(lib().hipns_slice_quotients(None, ptr(n), n.numel(), ptr(y), _stream()))
+        got = y.cpu().numpy()
+        nn = n.cpu().numpy()
+        want = np.float32(1.0) / nn
+        k = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+        if k.size:
+            say("binade", e, sign, "mismatches", k.size, "n", repr(nn[k[0]]), "got", repr(got[k[0]]),
+                "want", repr(want[k[0]]))
+        bad += k.size
+    one = torch.tensor([1.0, -1.0], dtype=torch.float32, device="cuda")
+    y1 = torch.empty(2, dtype=torch.float32, device="cuda")
+    check(lib().hipns_slice_quotients(None, ptr(one), 2, ptr(y1), _stream()))
+    assert y1.cpu().tolist() == [1.0, -1.0]
+    assert bad == 0
 
 
 def _evidence_inputs(M, seed):
