@@ -44,17 +44,19 @@ PEAK_FP32_MFMA = 157.3e12
 MEASURED_TRAFFIC = {("c2", 1024): 369.88e6, ("c4", 128): 17.68e6, ("c4", 256): 32.55e6}
 PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
-# wavefront VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 FP32 instruction
-# per 2 cycles per SIMD (the 157.3 TFLOP/s FP32 vector peak is 32 FMA lanes per SIMD per cycle)
-PEAK_VALU_ISSUE = 256 * 4 * 2.4e9 / 2.0
+# wavefront VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4
+# cycles per SIMD (16 lanes).  The 157.3 TFLOP/s FP32 vector peak counts packed FP32 (two FMAs per lane
+# per instruction), which these kernels are built without; the counters agree: SQ_ACTIVE_INST_VALU x 4
+# cycles per SQ_INSTS_VALU is 4.3 on the MVN sampling kernel
+PEAK_VALU_ISSUE = 256 * 4 * 2.4e9 / 4.0
 # SQ_INSTS_VALU per k_sample launch (rocprofv3 --pmc on the bench's instrumented pass of the same
 # shape), keyed like MEASURED_TRAFFIC
-MEASURED_VALU = {("c2", 1024): 1.1958e10}
+MEASURED_VALU = {("c2", 1024): 1.1067e10}
 # SQ_INSTS_VALU per likelihood evaluation of the sampling kernel (rocprofv3 --pmc on the instrumented
 # pass: mixture batch of 256 runs at 8 lanes per chain, MVN D=10 batch of 1024 runs at 2 lanes per chain),
 # keyed by (workload, lanes per chain); times a step's evaluations per second it gives the VALU issue
 # rate the step sustains (profiles/k_sample_gmm_mixture_r256.md)
-MEASURED_VALU_PER_EVAL = {("c3", 8): 80.76, ("c1", 2): 21.61}
+MEASURED_VALU_PER_EVAL = {("c3", 8): 76.44, ("c1", 2): 19.95}
 # the oracle ensemble of the same workload (tests/golden/oracle_logZ_ensembles.json) beside the
 # device's log Z statistics
 # runs of a step streamed through this many device slots (engine.run_queue) instead of one batch: none
@@ -782,7 +784,7 @@ def main():
     valu = MEASURED_VALU.get((args.config, RG))
     if valu is not None:
         # the kernel's VALU issue: SQ_INSTS_VALU per launch (rocprofv3 --pmc on the same launch shape) over the
-        # live average launch duration, against one wavefront instruction per 2 cycles per SIMD
+        # live average launch duration, against one wavefront instruction per 4 cycles per SIMD
         rate = valu / avg_dur
         roofline["valu_ceiling"] = dict(insts_per_launch=valu, achieved=rate, peak=PEAK_VALU_ISSUE,
                                         unit="wavefront VALU instructions/s", frac=rate / PEAK_VALU_ISSUE)
