@@ -647,17 +647,37 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
   for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) part[k] = 0.0f;
   float qv[NJ];
   prior_q<NJ>(up, qv, tfm);
+  // Slots 4i..4i+3 of a lane are four consecutive dimensions (owned_dim), so the prior scale, offset and
+  // component means come in 16-byte LDS reads, all issued before the arithmetic that waits on them.
+  // The reads cover the run's padding dimensions (below the row stride, so inside the LDS block);
+  // those slots contribute nothing.
+  float xv[NJ];
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) {
+  for (int j = 0; j < NJ; j += 4) {
     if (j < nj) {  // uniform per run
-      const int d = owned_dim<LPC>(h, j);
-      const bool in = d < D;  // padding slots contribute nothing
-      const float x = fmaf(P.q[d], qv[j], P.p[d]);
+      const int d0 = owned_dim<LPC>(h, j);
+      const float4 pp = *reinterpret_cast<const float4*>(P.p + d0);
+      const float4 qq = *reinterpret_cast<const float4*>(P.q + d0);
+      xv[j] = fmaf(qq.x, qv[j], pp.x);
+      xv[j + 1] = fmaf(qq.y, qv[j + 1], pp.y);
+      xv[j + 2] = fmaf(qq.z, qv[j + 2], pp.z);
+      xv[j + 3] = fmaf(qq.w, qv[j + 3], pp.w);
+    }
+  }
 #pragma unroll
-      for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
-        if (k < P.K) {
-          const float r = in ? x - P.mean[k * P.dps + d] : 0.0f;
-          part[k] = fmaf(r, r, part[k]);
+  for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
+    if (k < P.K) {
+#pragma unroll
+      for (int j = 0; j < NJ; j += 4) {
+        if (j < nj) {
+          const int d0 = owned_dim<LPC>(h, j);
+          const float4 m = *reinterpret_cast<const float4*>(P.mean + k * P.dps + d0);
+          const float mv[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float r = d0 + i < D ? xv[j + i] - mv[i] : 0.0f;  // padding slots contribute nothing
+            part[k] = fmaf(r, r, part[k]);
+          }
         }
       }
     }
@@ -698,7 +718,7 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
         const bool in = j < nj && d < D;
         float dx = 0.0f;
         if (in) {
-          const float x = fmaf(P.q[d], qv[j], P.p[d]);
+          const float x = xv[j];
 #pragma unroll
           for (int k = 0; k < HIPNS_MAX_GMM_K; ++k)
             if (k < P.K) dx = fmaf(wk[k], x - P.mean[k * P.dps + d], dx);
