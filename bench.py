@@ -797,10 +797,11 @@ def main():
     sec = {}
     if not args.no_secondary and args.config == "c2":
         # (line, workload, runs per step, timed steps, warm-up steps, concurrent groups): the mixture
-        # batch at the 256 runs of its configuration and at 2048 (its 12 s step runs cold: first-launch
-        # costs are below 1 % of it)
+        # batch at the 256 runs of its configuration, in four concurrent groups of 64 (one per hardware
+        # queue: 1 group 153 runs/s, 2 168, 4 175, 8 109 on one box), and at 2048 in one (219 runs/s; 4
+        # groups 215), whose 9 s step runs cold: first-launch costs are below 1 % of it
         for key, cfg, Rs, st, wu, gr in (("c1", "c1", 1024, 2, 1, 1), ("c4", "c4", 256, 1, 1, 2),
-                                         ("c3", "c3", 256, 2, 1, 1), ("c3_2048", "c3", 2048, 1, 0, 1)):
+                                         ("c3", "c3", 256, 2, 1, 4), ("c3_2048", "c3", 2048, 1, 0, 1)):
             t0 = time.perf_counter()
             sec[key] = secondary(cfg, Rs, st, args.graph_iters, slots=QUEUE_SLOTS.get(cfg, 0), groups=gr, warmup=wu)
             log(f"secondary {key}: {sec[key]['runs_per_s']:.2f} runs/s in {time.perf_counter() - t0:.1f} s")
