@@ -665,14 +665,19 @@ __device__ __forceinline__ double lik_gmm(const float (&up)[NJ], int nj, int D, 
     }
   }
 #pragma unroll
-  for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
-    if (k < P.K) {
+  for (int j = 0; j < NJ; j += 4) {
+    if (j < nj) {
+      const int d0 = owned_dim<LPC>(h, j);
+      // every component's four means of these slots, read before any is used (components past K
+      // read component K - 1 again, a valid row, and are not used)
+      float4 m[HIPNS_MAX_GMM_K];
 #pragma unroll
-      for (int j = 0; j < NJ; j += 4) {
-        if (j < nj) {
-          const int d0 = owned_dim<LPC>(h, j);
-          const float4 m = *reinterpret_cast<const float4*>(P.mean + k * P.dps + d0);
-          const float mv[4] = {m.x, m.y, m.z, m.w};
+      for (int k = 0; k < HIPNS_MAX_GMM_K; ++k)
+        m[k] = *reinterpret_cast<const float4*>(P.mean + (k < P.K ? k : P.K - 1) * P.dps + d0);
+#pragma unroll
+      for (int k = 0; k < HIPNS_MAX_GMM_K; ++k) {
+        if (k < P.K) {
+          const float mv[4] = {m[k].x, m[k].y, m[k].z, m[k].w};
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const float r = d0 + i < D ? xv[j + i] - mv[i] : 0.0f;  // padding slots contribute nothing

@@ -15,7 +15,10 @@ error.  Workloads:
 Each oracle run uses its own stream (seed 7_000_000 + index); the device uses Philox streams, so
 the runs are statistically, not bitwise, matched.
 
-  python tests/golden/make_oracle_ensembles.py [workload ...] [--procs 8]
+  python tests/golden/make_oracle_ensembles.py [workload ...] [--procs 8] [--extend K]
+
+--extend K adds K runs (the next K indices) to a workload already in the file and recomputes its
+summary over all of its runs.
 """
 import argparse
 import json
@@ -89,17 +92,21 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("workloads", nargs="*", default=list(WORKLOADS))
     ap.add_argument("--procs", type=int, default=8)
+    ap.add_argument("--extend", type=int, default=0)
     a = ap.parse_args()
     data = json.load(open(OUT)) if os.path.exists(OUT) else {}
     for name in a.workloads:
-        n = WORKLOADS[name]["runs"]
+        old = data.get(name, {}).get("runs", []) if a.extend else []
+        first = max((r["i"] for r in old), default=-1) + 1
+        n = a.extend if a.extend else WORKLOADS[name]["runs"]
         t0 = time.perf_counter()
         rows = []
         with Pool(a.procs) as pool:
-            for r in pool.imap_unordered(_one, [(name, i) for i in range(n)]):
+            for r in pool.imap_unordered(_one, [(name, i) for i in range(first, first + n)]):
                 rows.append(r)
                 if len(rows) % 20 == 0:
                     print(f"{name}: {len(rows)}/{n} runs, {time.perf_counter() - t0:.0f} s", flush=True)
+        rows = list(old) + rows
         rows.sort(key=lambda r: r["i"])
         s = summarise(rows)
         data[name] = dict(summary=s, runs=[dict(i=r["i"], D=r["D"], N=r["N"], dlogZ=round(r["dlogZ"], 6),
