@@ -11,7 +11,8 @@ enum SampleMode : int {
   MODE_UNIF = 1,   // UniformSampler rejection chains of one shell
   MODE_INIT = 2,   // initial live points: uniform until logL > -inf
   MODE_INJ = 3,    // slice chains with injected draws and a recorded trace
-  MODE_GSLICE = 4, // slice chains whose next direction reflects the last one about grad logL (stepwise only)
+  MODE_GSLICE = 4, // slice chains that use grad logL: in the fused kernel gradient_guided reflections and / or
+                   // gradient_slice directions (run codes sampler 2-4), in the stepwise form gradient_guided
 };
 
 struct SampleArgs {
