@@ -56,7 +56,7 @@ MEASURED_VALU = {("c2", 1024): 1.1067e10}
 # pass: mixture batch of 256 runs at 8 lanes per chain, MVN D=10 batch of 1024 runs at 2 lanes per chain),
 # keyed by (workload, lanes per chain); times a step's evaluations per second it gives the VALU issue
 # rate the step sustains (profiles/k_sample_gmm_mixture_r256.md)
-MEASURED_VALU_PER_EVAL = {("c3", 8): 79.21, ("c1", 2): 19.95}
+MEASURED_VALU_PER_EVAL = {("c3", 8): 76.31, ("c1", 2): 19.95}
 # the oracle ensemble of the same workload (tests/golden/oracle_logZ_ensembles.json) beside the
 # device's log Z statistics
 # runs of a step streamed through this many device slots (engine.run_queue) instead of one batch: none
