@@ -457,6 +457,36 @@ def test_batched_runs_match_single_runs():
     assert a.log_Z_mean == b.log_Z_mean
 
 
+def test_mixture_runs_do_not_depend_on_lane_layout():
+    """A mixture run's samples and log Z do not depend on the lanes per chain. The draws are keyed by
+    chain, shell, slice and dimension, min / max reductions do not depend on order, and the
+    squared-distance partial sums meet in the same pairing tree at every layout. That holds for runs
+    of up to 16 dimensions at 2, 4 and 8 lanes per chain and up to 32 at 4 and 8. A run of D = 40 in
+    the batch gives the U rows a stride of 64, which 8 lanes per chain need."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    say("mixture runs across lane layouts")
+    Ds = [5, 12, 16, 27, 40]
+    models = [gmm_problem(D, seed=70 + D)[0] for D in Ds]
+    term = dict(dlogZ=math.log(1.0 + 1e-2), max_samples=1e18)
+    out = {}
+    for lpc in (2, 4, 8):
+        specs = [RunSpec(model=m, num_live_points=96, num_slices=3 * D, max_samples=96 * 150, seed=40 + i)
+                 for i, (m, D) in enumerate(zip(models, Ds))]
+        b = DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=2)
+        b.run(term, graph_iters=4)
+        assert int(b.cap_hits().sum()) == 0
+        out[lpc] = [b.to_results(r) for r in range(b.R)]
+    for lpcs, runs in (((2, 4, 8), (0, 1, 2)), ((4, 8), (3,))):
+        for r in runs:
+            a = out[lpcs[0]][r]
+            for lpc in lpcs[1:]:
+                c = out[lpc][r]
+                assert a.total_num_samples == c.total_num_samples, (Ds[r], lpc)
+                assert np.array_equal(a.log_L_samples, c.log_L_samples), (Ds[r], lpc)
+                assert np.array_equal(a.U_samples, c.U_samples), (Ds[r], lpc)
+                assert a.log_Z_mean == c.log_Z_mean, (Ds[r], lpc)
+
+
 def test_concurrent_batches_match_sequential():
     """engine.run_concurrent (two batches on streams of their own, their shell graphs replayed in turn)
     gives every run the same samples, evaluation counts and log Z as DeviceBatch.run of each batch
