@@ -653,6 +653,22 @@ def secondary(cfg, R, steps, graph_iters, slots=0, groups=1, warmup=1):
     return out
 
 
+def secondary_in_child(key, cfg, R, steps, warmup, groups, graph_iters):
+    """secondary() in a child process (python bench.py --emit-secondary): the line's JSON result."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--config", cfg, "--runs", str(R), "--steps", str(steps),
+           "--warmup", str(warmup), "--groups", str(groups), "--graph-iters", str(graph_iters),
+           "--emit-secondary", key]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=600)
+    for ln in p.stdout.splitlines():
+        if not ln.startswith('{"emit_secondary"'):
+            print(ln, file=sys.stderr, flush=True)  # the child's autotuner lines
+    if p.returncode != 0:
+        raise RuntimeError(f"secondary {key}: child exited with {p.returncode}")
+    lines = [ln for ln in p.stdout.splitlines() if ln.startswith('{"emit_secondary"')]
+    return json.loads(lines[-1])["result"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -675,6 +691,8 @@ def main():
     ap.add_argument("--lr-terms", type=int, default=3, choices=[3, 4])
     ap.add_argument("--plain-runs", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--emit-secondary", default=None,
+                    help="print one secondary line of --config (runs, steps, warmup, groups as given) as JSON")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the one-step measurements of the other configurations")
     ap.add_argument("--no-autotune", action="store_true", help="use --lpc/--wpb instead of the autotuner")
@@ -703,6 +721,11 @@ def main():
         log(f"copy calibration: 3 x {args.copy_calib} MiB read + written")
     if args.to_results_scale:
         print(json.dumps(to_results_scale(args.to_results_scale, args.steps)), flush=True)
+        return
+    if args.emit_secondary:
+        res = secondary(args.config, args.runs, args.steps, args.graph_iters, slots=QUEUE_SLOTS.get(args.config, 0),
+                        groups=max(1, args.groups), warmup=args.warmup)
+        print(json.dumps({"emit_secondary": args.emit_secondary, "result": res}), flush=True)
         return
     wl = make_workload(args.config)
     if args.config == "c4":
@@ -800,11 +823,30 @@ def main():
         # batch at the 256 runs of its configuration, in four concurrent groups of 64 (one per hardware
         # queue: 1 group 153 runs/s, 2 168, 4 175, 8 109 on one box), and at 2048 in one (219 runs/s; 4
         # groups 215), whose 9 s step runs cold: first-launch costs are below 1 % of it
-        for key, cfg, Rs, st, wu, gr in (("c1", "c1", 1024, 2, 1, 1), ("c4", "c4", 256, 1, 1, 2),
+        # Each line runs in a child process of its own, as the workload would run alone: in this process,
+        # after the headline's batches, the mixture batch's second step ran 38 % slower (1.85 s against
+        # 1.33 s for the same runs in a fresh process)
+        for key, cfg, Rs, st, wu, gr in (("c1", "c1", 1024, 8, 1, 1), ("c4", "c4", 256, 1, 1, 2),
                                          ("c3", "c3", 256, 2, 1, 4), ("c3_2048", "c3", 2048, 1, 0, 1)):
             t0 = time.perf_counter()
-            sec[key] = secondary(cfg, Rs, st, args.graph_iters, slots=QUEUE_SLOTS.get(cfg, 0), groups=gr, warmup=wu)
+            sec[key] = secondary_in_child(key, cfg, Rs, st, wu, gr, args.graph_iters)
             log(f"secondary {key}: {sec[key]['runs_per_s']:.2f} runs/s in {time.perf_counter() - t0:.1f} s")
+        # precision at scale (65 536 x 128 regression): three shells of 8 partial runs in each product form
+        # (a whole run needs ~2 100 shells), their evaluation rate and the accepted points' logL error
+        # against the FP64 likelihood
+        t0 = time.perf_counter()
+        wl5 = make_workload("c5")
+        res5 = precision_at_scale(wl5, 8, 3, 4, 4, 1)
+        bf, fp = res5["bf16x3"], res5["fp32"]
+        sec["c5"] = dict(workload=wl5.name, runs_per_step=8, shells=3, evals_per_s=bf["evals_per_s"],
+                         plain_fp32_evals_per_s=fp["evals_per_s"], bf16x3_vs_fp32=bf["evals_per_s"] / fp["evals_per_s"],
+                         logL_rel_err_max=bf["logL_rel_err_max"],
+                         unrefined_logL_rel_err_max=bf.get("unrefined_logL_rel_err_max"),
+                         plain_fp32_logL_rel_err_max=fp["logL_rel_err_max"],
+                         roofline=dict(bound="mfma", achieved=bf["achieved_TFLOPs"], peak=PEAK_BF16_MFMA / 1e12,
+                                       unit="TFLOP/s", frac=bf["achieved_TFLOPs"] * 1e12 / PEAK_BF16_MFMA,
+                                       kernel=wl5.kernel, avg_launch_ms=bf["avg_launch_ms"]))
+        log(f"secondary c5: {bf['evals_per_s'] / 1e6:.1f} M evals/s in {time.perf_counter() - t0:.1f} s")
     out = {
         "metric": "runs/s to |dlogZ|<0.1 nat (chosen batched form, 1 GPU)",
         "value": runs_per_s,
