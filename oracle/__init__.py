@@ -87,7 +87,7 @@ class Register(C.Structure):
 class RunCfg(C.Structure):
     _fields_ = [("num_live_points", C.c_int), ("num_slices", C.c_int), ("k_phantom", C.c_int),
                 ("midpoint_shrink", C.c_int), ("max_samples", _i64), ("init_efficiency_threshold", _d),
-                ("seed", C.c_uint64), ("num_threads", C.c_int), ("max_iterations", _i64)]
+                ("seed", C.c_uint64), ("num_threads", C.c_int), ("max_iterations", _i64), ("dir16", C.c_int)]
 
 
 class RunResult(C.Structure):
@@ -394,10 +394,11 @@ def init_register() -> Register:
 
 
 def ns_run(spec: ModelSpec, *, num_live_points, num_slices, k_phantom=0, midpoint_shrink=True, max_samples,
-           seed=0, num_threads=0, max_iterations=-1, init_efficiency_threshold=0.1, term=None):
-    """Full nested-sampling run (create_init_state -> phases -> to_results) on host cores."""
+           seed=0, num_threads=0, max_iterations=-1, init_efficiency_threshold=0.1, term=None, dir16=False):
+    """Full nested-sampling run (create_init_state -> phases -> to_results) on host cores.  dir16: slice
+    directions from 16-bit uniform pairs as the device draws them, instead of FP64 Box-Muller."""
     cfg = RunCfg(num_live_points, num_slices, k_phantom, int(midpoint_shrink), int(max_samples),
-                 float(init_efficiency_threshold), int(seed), int(num_threads), int(max_iterations))
+                 float(init_efficiency_threshold), int(seed), int(num_threads), int(max_iterations), int(dir16))
     if term is None:
         term = dict(dlogZ=np.log(1.0 + 1e-3), max_samples=float(max_samples))
     tc = make_term_cond(**term)

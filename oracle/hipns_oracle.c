@@ -634,6 +634,7 @@ int orc_determine_termination(const orc_term_cond_t* c, const orc_register_t* r,
 
 typedef struct {
   uint32_t key[2];
+  int dir16;
 } orc_rng_t;
 
 static double u01_from(uint32_t a, uint32_t b) { /* 53-bit uniform in [0,1) */
@@ -668,9 +669,26 @@ static void sample_direction(const orc_rng_t* g, int D, uint32_t chain, uint32_t
     return;
   }
   double ss = 0.0;
-  for (int i = 0; i < D; ++i) {
-    n[i] = rng_normal(g, chain, it, slice, (1u << 24) | (uint32_t)i);
-    ss += n[i] * n[i];
+  if (g->dir16) {
+    /* the device's form: each 32-bit word of a Philox block gives u1 = (low 16 bits + 1) / 2^16 in
+     * (0, 1] and u2 = (high 16 bits) / 2^16, and the pair r cos(2 pi u2), r sin(2 pi u2) */
+    for (int i0 = 0; i0 < D; i0 += 8) {
+      uint32_t w[4];
+      rng_block(g, chain, it, slice, (3u << 24) | (uint32_t)(i0 >> 3), w);
+      for (int q = 0; q < 4; ++q) {
+        double u1 = (double)((w[q] & 0xFFFFu) + 1u) / 65536.0;
+        double u2 = (double)(w[q] >> 16) / 65536.0;
+        double r = sqrt(-2.0 * log(u1));
+        if (i0 + 2 * q < D) n[i0 + 2 * q] = r * cos(2.0 * M_PI * u2);
+        if (i0 + 2 * q + 1 < D) n[i0 + 2 * q + 1] = r * sin(2.0 * M_PI * u2);
+      }
+    }
+    for (int i = 0; i < D; ++i) ss += n[i] * n[i];
+  } else {
+    for (int i = 0; i < D; ++i) {
+      n[i] = rng_normal(g, chain, it, slice, (1u << 24) | (uint32_t)i);
+      ss += n[i] * n[i];
+    }
   }
   double nrm = sqrt(ss);
   for (int i = 0; i < D; ++i) n[i] /= nrm;
@@ -843,6 +861,7 @@ int orc_ns_run(const orc_model_t* m, const orc_run_cfg_t* cfg, const orc_term_co
   orc_rng_t g;
   g.key[0] = (uint32_t)cfg->seed;
   g.key[1] = (uint32_t)(cfg->seed >> 32);
+  g.dir16 = cfg->dir16;
 #ifdef _OPENMP
   int nthr = cfg->num_threads > 0 ? cfg->num_threads : omp_get_max_threads();
 #endif

@@ -157,6 +157,9 @@ typedef struct {
   uint64_t seed;
   int num_threads;           /* OpenMP threads over the chains of a shell, <=0 => all */
   int64_t max_iterations;    /* stop early after this many shells (<0: none) -- for bounded timing samples */
+  int dir16;                 /* 0: slice directions from FP64 Box-Muller per dimension; 1: from 16-bit
+                                uniform pairs, eight dimensions per Philox block, as the device draws
+                                them (a variant for comparing the two, not the reference's) */
 } orc_run_cfg_t;
 
 typedef struct {

@@ -12,6 +12,8 @@ error.  Workloads:
             means and widths drawn from the seed), one oracle run per specification
   linreg_s10, linreg_s20   the regression workload with s = 10 and 20 slices per dimension (200 runs
             each): how the algorithm's log Z bias at s = 5 changes with longer chains
+  gmm_dir16 the mixture runs of gmm with the slice directions drawn as the device draws them (16-bit
+            uniform pairs), to compare the two direction forms
 Each oracle run uses its own stream (seed 7_000_000 + index); the device uses Philox streams, so
 the runs are statistically, not bitwise, matched.
 
@@ -43,6 +45,7 @@ WORKLOADS = {
     "gmm": dict(runs=256),
     "linreg_s10": dict(runs=200),
     "linreg_s20": dict(runs=200),
+    "gmm_dir16": dict(runs=1024),
 }
 
 
@@ -58,7 +61,7 @@ def _setup(name, i):
         _, log_Z, lik = P.linreg_problem(4096, 16, seed=0, sigma=0.3, tau=1.0)
         spec, const = P.linreg_mvn_oracle_spec(lik, 1.0)
         return spec, 1024, 16 * s, 1024 * 200, log_Z, const, 16
-    if name == "gmm":
+    if name in ("gmm", "gmm_dir16"):
         rng = np.random.default_rng(i)
         D = int(rng.integers(4, 65))
         N = int(rng.integers(128, 1025))
@@ -72,7 +75,8 @@ def _one(args):
     name, i = args
     spec, N, ns, ms, log_Z, const, D = _setup(name, i)
     t0 = time.perf_counter()
-    r = O.ns_run(spec, num_live_points=N, num_slices=ns, max_samples=ms, seed=7_000_000 + i, num_threads=1)
+    r = O.ns_run(spec, num_live_points=N, num_slices=ns, max_samples=ms, seed=7_000_000 + i, num_threads=1,
+                 dir16=name.endswith("_dir16"))
     return dict(i=i, D=D, N=N, dlogZ=r["log_Z_mean"] + const - log_Z, uncert=r["log_Z_uncert"],
                 evals=int(r["total_likelihood_evaluations"]), reason=int(r["termination_reason"]),
                 wall_s=time.perf_counter() - t0)
