@@ -41,7 +41,7 @@ PEAK_FP32_MFMA = 157.3e12
 # instrumented batch (counters scaled by a 256 MiB copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
 # profiles/k_sample_mvn_d32_n2048_r1024.md and profiles/k_sample_linreg_4096x16_n1024_r128.md.
 # Keyed by (workload, runs per launch); other shapes report null.
-MEASURED_TRAFFIC = {("c2", 1024): 369.88e6, ("c4", 128): 17.68e6, ("c4", 256): 32.55e6}
+MEASURED_TRAFFIC = {("c2", 1024): 370.0e6, ("c4", 128): 17.68e6, ("c4", 256): 32.55e6}
 PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
 # wavefront VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4
@@ -51,12 +51,12 @@ PEAK_HBM = 8.0e12
 PEAK_VALU_ISSUE = 256 * 4 * 2.4e9 / 4.0
 # SQ_INSTS_VALU per k_sample launch (rocprofv3 --pmc on the bench's instrumented pass of the same
 # shape), keyed like MEASURED_TRAFFIC
-MEASURED_VALU = {("c2", 1024): 1.1067e10}
+MEASURED_VALU = {("c2", 1024): 1.1726e10}
 # SQ_INSTS_VALU per likelihood evaluation of the sampling kernel (rocprofv3 --pmc on the instrumented
 # pass: mixture batch of 256 runs at 8 lanes per chain, MVN D=10 batch of 1024 runs at 2 lanes per chain),
 # keyed by (workload, lanes per chain); times a step's evaluations per second it gives the VALU issue
 # rate the step sustains (profiles/k_sample_gmm_mixture_r256.md)
-MEASURED_VALU_PER_EVAL = {("c3", 8): 76.31, ("c1", 2): 19.95}
+MEASURED_VALU_PER_EVAL = {("c3", 8): 82.58, ("c1", 2): 20.90}
 # the oracle ensemble of the same workload (tests/golden/oracle_logZ_ensembles.json) beside the
 # device's log Z statistics
 # runs of a step streamed through this many device slots (engine.run_queue) instead of one batch: none

@@ -168,6 +168,7 @@ enum : uint32_t {
   RNG_DIR = 1u << 24,
   RNG_PICK = 2u << 24,
   RNG_UNIF = 3u << 24,
+  RNG_DIR2 = 4u << 24,  // the angle uniforms of the slice directions
 };
 
 // ---------------------------------------------------------------------------

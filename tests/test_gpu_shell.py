@@ -340,15 +340,17 @@ def test_seed_indices_match_oracle(N, first):
 
 
 def _oracle_direction(k0, k1, it, chain, si, D):
-    """16-bit Box-Muller normals from the chain's Philox blocks (8 dimensions per block), normalised."""
+    """Box-Muller normals from two Philox blocks per 8 dimensions (radius uniforms from the first,
+    angle uniforms from the second, 24 bits each), normalised."""
     if D == 1:
         return np.ones(1)
     z = np.zeros(((D + 7) // 8) * 8)
     for blk in range(z.size // 8):
         w = O.philox4x32([chain, it, si, (1 << 24) | blk], [k0, k1])
+        v = O.philox4x32([chain, it, si, (4 << 24) | blk], [k0, k1])
         for q in range(4):
-            u1 = ((int(w[q]) & 0xFFFF) + 1) / 65536.0
-            u2 = (int(w[q]) >> 16) / 65536.0
+            u1 = ((int(w[q]) >> 8) + 1) / 16777216.0
+            u2 = (int(v[q]) >> 8) / 16777216.0
             rr = math.sqrt(-2.0 * math.log(u1))
             z[8 * blk + 2 * q] = rr * math.cos(2 * math.pi * u2)
             z[8 * blk + 2 * q + 1] = rr * math.sin(2 * math.pi * u2)
@@ -359,7 +361,7 @@ def _oracle_direction(k0, k1, it, chain, si, D):
 @pytest.mark.parametrize("D,lpc", [(1, 2), (3, 4), (10, 2), (32, 4), (33, 2), (70, 4), (128, 2)])
 def test_slice_directions(D, lpc):
     """_sample_direction (uni_slice_sampler.py:23-38): unit vectors, equal to the FP64 restatement of
-    the kernel's 16-bit Box-Muller draws to 2e-5 per component (hardware log2 / sin / cos), and
+    the kernel's Box-Muller draws (24-bit uniforms) to 2e-5 per component (hardware log2 / sin / cos), and
     isotropic: n_1^2 follows Beta(1/2, (D-1)/2) (KS test over 24 576 directions)."""
     from scipy import stats
     torch = _torch()
