@@ -38,15 +38,16 @@ def _key(specs, lr_terms) -> str:
     if fam == 2:
         extra = f"-rows{lik.N}-t{lr_terms}"
     h = hashlib.sha1(repr(shape).encode()).hexdigest()[:12]
-    return f"fam{fam}{extra}-R{len(specs)}-{h}-{name}-c3"
+    return f"fam{fam}{extra}-R{len(specs)}-{h}-{name}-c4"
 
 
-def measure(specs, lpc, wpb, shells=3, lr_terms=3) -> float:
-    """Likelihood evaluations per second of the sampling kernel over `shells` shells."""
+def measure(specs, lpc, wpb, shells=3, lr_terms=3, min_time=0.02, max_shells=12) -> float:
+    """Likelihood evaluations per second of the sampling kernel over `shells` shells, or more (up to
+    max_shells) until the timed kernel time reaches min_time seconds: short shells time noisily."""
     import torch
     from ._lib import check, lib, make_term_cond, ptr, stream_ptr
     from .engine import DeviceBatch
-    trial = [dataclasses.replace(s, max_samples=s.num_live_points * (shells + 4), seed=777 + i)
+    trial = [dataclasses.replace(s, max_samples=s.num_live_points * (max(shells, max_shells) + 4), seed=777 + i)
              for i, s in enumerate(specs)]
     b = DeviceBatch(trial, lanes_per_chain=lpc, waves_per_block=wpb, lr_terms=lr_terms)
     b.init_state()
@@ -55,7 +56,9 @@ def measure(specs, lpc, wpb, shells=3, lr_terms=3) -> float:
     t = 0.0
     evals = 0
     prev = sum(st.num_likelihood_evaluations for st in b.states())
-    for _ in range(shells):
+    done = 0
+    while done < shells or (t < min_time and done < max_shells):
+        done += 1
         check(lib().hipns_shell_discard(ptr(b.runs_dev), ptr(b.states_dev), b.R, b.runs_host, C.byref(b.bufs), s))
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
