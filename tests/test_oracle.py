@@ -277,6 +277,26 @@ def test_oracle_full_run_analytic():
     assert abs(zs[:, 0].mean() - log_Z) < 3 * zs[:, 1].mean() / 2 + 0.05
 
 
+def test_oracle_full_run_device_direction_form():
+    """The run option dir16 (slice directions from 16-bit uniform pairs, eight dimensions per Philox
+    block, the form the device used to draw) changes the run's random stream, and its log Z agrees with
+    the analytic value as the FP64 Box-Muller form's does."""
+    from problems import mvn_problem
+    _, log_Z, mu, Sig = mvn_problem(6, seed=4)
+    spec = mvn_oracle_spec(6, mu, Sig)
+    zs = []
+    for seed in range(4):
+        a = O.ns_run(spec, num_live_points=200, num_slices=30, max_samples=200 * 100, seed=seed, num_threads=4)
+        b = O.ns_run(spec, num_live_points=200, num_slices=30, max_samples=200 * 100, seed=seed, num_threads=4,
+                     dir16=True)
+        assert a["total_likelihood_evaluations"] != b["total_likelihood_evaluations"] or \
+            not np.array_equal(a["samples_logL"], b["samples_logL"])
+        assert b["termination_reason"] & 4
+        zs.append((b["log_Z_mean"], b["log_Z_uncert"]))
+    zs = np.array(zs)
+    assert abs(zs[:, 0].mean() - log_Z) < 3 * zs[:, 1].mean() / 2 + 0.05
+
+
 def test_linreg_loglik_vs_scipy():
     """Oracle regression likelihood at x(u) = tau ndtri(u) vs scipy's normal logpdf summed over rows."""
     _, _, lik = linreg_problem(200, 7, seed=3, sigma=0.5, tau=2.0)
