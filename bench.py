@@ -53,10 +53,11 @@ PEAK_VALU_ISSUE = 256 * 4 * 2.4e9 / 4.0
 # shape), keyed like MEASURED_TRAFFIC
 MEASURED_VALU = {("c2", 1024): 1.1726e10}
 # SQ_INSTS_VALU per likelihood evaluation of the sampling kernel (rocprofv3 --pmc on the instrumented
-# pass: mixture batch of 256 runs at 8 lanes per chain, MVN D=10 batch of 1024 runs at 2 lanes per chain),
-# keyed by (workload, lanes per chain); times a step's evaluations per second it gives the VALU issue
-# rate the step sustains (profiles/k_sample_gmm_mixture_r256.md)
-MEASURED_VALU_PER_EVAL = {("c3", 8): 82.58, ("c1", 2): 20.90}
+# pass: mixture batch of 256 runs at 8 lanes per chain and one wavefront per workgroup, MVN D=10 batch of
+# 1024 runs at 2 lanes per chain and one wavefront per workgroup), keyed by (workload, lanes per chain);
+# times a step's evaluations per second it gives the VALU issue rate the step sustains
+# (profiles/k_sample_gmm_mixture_r256.md, profiles/k_sample_mvn_d10_n300_r1024.md)
+MEASURED_VALU_PER_EVAL = {("c3", 8): 82.58, ("c1", 2): 21.74}
 # the oracle ensemble of the same workload (tests/golden/oracle_logZ_ensembles.json) beside the
 # device's log Z statistics
 # runs of a step streamed through this many device slots (engine.run_queue) instead of one batch: none
