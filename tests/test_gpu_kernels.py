@@ -487,6 +487,39 @@ def test_mixture_runs_do_not_depend_on_lane_layout():
                 assert a.log_Z_mean == c.log_Z_mean, (Ds[r], lpc)
 
 
+def test_runs_do_not_depend_on_waves_per_block():
+    """The autotuner picks the wavefronts per workgroup for each batch; that choice only moves chains
+    between workgroups (a run's chains fill whole workgroups, the slots beyond num_live_points / 2
+    stay empty), so every run's samples, evaluation counts and log Z are bit-identical at 1, 2 and 4
+    wavefronts per workgroup. MVN runs with 150 chains (5, 3 and 2 workgroups of 32, 64 and 128 chain
+    slots at 2 lanes per chain) and a mixture batch at 8 lanes per chain."""
+    from jaxns_amd.engine import DeviceBatch, RunSpec
+    say("runs across waves per workgroup")
+    mvn, _, _, _ = mvn_problem(10, seed=21)
+    mix = [gmm_problem(D, seed=80 + D)[0] for D in (6, 23)]
+    term = dict(dlogZ=math.log(1.0 + 1e-2), max_samples=1e18)
+    cases = [("mvn", [RunSpec(model=mvn, num_live_points=300, num_slices=20, max_samples=300 * 120, seed=60 + i)
+                      for i in range(3)], 2),
+             ("gmm", [RunSpec(model=m, num_live_points=100 + 60 * i, num_slices=2 * m.U_ndims,
+                              max_samples=220 * 150, seed=70 + i) for i, m in enumerate(mix)], 8)]
+    for name, specs, lpc in cases:
+        out = {}
+        for wpb in (1, 2, 4):
+            b = DeviceBatch(specs, lanes_per_chain=lpc, waves_per_block=wpb)
+            b.run(term, graph_iters=4)
+            assert int(b.cap_hits().sum()) == 0
+            out[wpb] = [b.to_results(r) for r in range(b.R)]
+        for r in range(len(specs)):
+            a = out[1][r]
+            for wpb in (2, 4):
+                c = out[wpb][r]
+                assert a.total_num_samples == c.total_num_samples, (name, r, wpb)
+                assert a.total_num_likelihood_evaluations == c.total_num_likelihood_evaluations, (name, r, wpb)
+                assert np.array_equal(a.log_L_samples, c.log_L_samples), (name, r, wpb)
+                assert np.array_equal(a.U_samples, c.U_samples), (name, r, wpb)
+                assert a.log_Z_mean == c.log_Z_mean, (name, r, wpb)
+
+
 def test_concurrent_batches_match_sequential():
     """engine.run_concurrent (two batches on streams of their own, their shell graphs replayed in turn)
     gives every run the same samples, evaluation counts and log Z as DeviceBatch.run of each batch
