@@ -706,8 +706,9 @@ def main():
                          "(counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this build has one GPU; "
-              "multi-GPU runs would shard independent runs across processes", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): the runs of a step are independent, so several GPUs "
+              "would each take a share of them in a process of their own, and that form is not measured",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
